@@ -1,0 +1,219 @@
+"""Throughput of the FX-correlation hot path on one MI355X.
+
+Default workload: 6 stations x 2 pols of 2-bit real samples at 128 MS/s,
+2048-point FFTs -> 1024 channels, 62 500 FFTs (1 s subintegration), FP64
+delay model, FP32 visibilities of 15 baselines + 6 autos x 4 products.
+Synthetic input following the signal model is generated directly in HBM
+(gcorr_amd.synth), so the timed region starts with the input resident.
+
+One step = one subintegration through FxKernel::process (gxc_process):
+F-stage (unpack, fringe rotation, FFT, fractional-sample correction),
+X-stage (cross-multiply-accumulate), finalise (normalise).
+
+Prints one JSON line.  roofline: the dominant kernel's algorithmic bytes per
+launch / its mean launch time (HIP events on the library's stream), against
+the 8 TB/s HBM peak.  cpu_baseline: the oracle (CPU restatement of fxkernel)
+timed on this host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Input Gsamples/s (all stations) + % HBM roofline; FP16-MFMA vs FP32 @1e-3 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp8"])
+    ap.add_argument("--numffts", type=int, default=0, help="override FFTs per subint (testing)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target wall time of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-h2d", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(workload, precision):
+    """HBM bytes per F-stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), committed as profiles/traffic_<workload>_<precision>.json."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (workload, precision))
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def cpu_baseline(w, poly, offs, target_s):
+    import numpy as np
+    from oracle import fxref
+    nthreads = max(1, min(os.cpu_count() or 1, 16))
+    cfgf = lambda n: fxref.make_config(w.nant, w.nchan, n, w.lo, w.bw, nbit=w.nbit,
+                                       is_complex=w.is_complex)
+    # calibrate on one thread, then size each thread's subint to the target
+    probe = max(8, min(w.numffts, 64))
+    rng = np.random.default_rng(1)
+    d = fxref.derive(cfgf(probe))
+    data = [rng.integers(0, 256, d.bytes_per_station, dtype=np.uint8) for _ in range(w.nant)]
+    t1 = fxref.bench(cfgf(probe), data, poly, offs, 1, 1)
+    per_fft = t1 / probe
+    n = int(max(probe, min(w.numffts, target_s / per_fft)))
+    d = fxref.derive(cfgf(n))
+    data = [rng.integers(0, 256, d.bytes_per_station, dtype=np.uint8) for _ in range(w.nant)]
+    secs = fxref.bench(cfgf(n), data, poly, offs, nthreads, 1)
+    samples = nthreads * w.nant * 2 * n * w.fftchannels
+    single = w.nant * 2 * probe * w.fftchannels / t1
+    return {
+        "value": samples / secs / 1e9,
+        "unit": "Gsamples/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": ("oracle FP32 fxkernel restatement, %d threads x one subint of %d FFTs "
+                   "(%s shape), own input copy per thread; single-thread %.4f Gsamples/s"
+                   % (nthreads, n, w.name, single / 1e9)),
+        "seconds": secs,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build's path runs on a single MI355X "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import gcorr_amd
+    from gcorr_amd import synth
+
+    w = synth.WORKLOADS[args.workload]
+    if args.numffts:
+        w = w.scaled(args.numffts)
+    dev = torch.device("cuda:0")
+    t = time.time()
+    bufs, poly, offs = synth.generate(w, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    print("generated %s in %.1f s" % (w.name, gen_s), file=sys.stderr, flush=True)
+
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                           precision=args.precision, autos=True)
+    k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
+    k.setDelays(poly, offs)
+    for _ in range(args.warmup):
+        k.process()
+    k.set_profiling(True)
+    f_ms = x_ms = fin_ms = 0.0
+    f_launch = x_launch = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        k.process()
+        tm = k.timings()
+        f_ms += tm["fstage_ms"]
+        x_ms += tm["xstage_ms"]
+        fin_ms += tm["finalise_ms"]
+        f_launch += tm["fstage_launches"]
+        x_launch += tm["xstage_launches"]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    k.set_profiling(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = w.input_samples / (ms_per_step * 1e-3) / 1e9
+
+    # algorithmic bytes per launch: input bits + nchan-bin spectra (F);
+    # spectra read once + visibilities written (X)
+    spec_bytes_per_sample = {"fp32": 4.0, "fp16": 2.0, "bf16": 2.0, "fp8": 1.0}[args.precision]
+    in_bytes_per_sample = 2.0 if w.is_complex else 0.25
+    if w.is_complex:
+        spec_bytes_per_sample *= 2
+    nrows = gcorr_amd.num_rows(w.nant, autos=True)
+    vis_bytes = nrows * 4 * w.nchan * 8
+    f_bytes = w.input_samples * (in_bytes_per_sample + spec_bytes_per_sample) / max(1, f_launch / args.steps)
+    x_bytes = (w.input_samples * spec_bytes_per_sample) / max(1, x_launch / args.steps) + vis_bytes
+    f_avg = f_ms / max(1, f_launch)
+    x_avg = x_ms / max(1, x_launch)
+    stages = {
+        "fstage": {"ms_per_launch": f_avg, "launches_per_step": f_launch / args.steps,
+                   "bytes_per_launch": f_bytes, "GBps": f_bytes / (f_avg * 1e-3) / 1e9 if f_avg else None},
+        "xstage": {"ms_per_launch": x_avg, "launches_per_step": x_launch / args.steps,
+                   "bytes_per_launch": x_bytes, "GBps": x_bytes / (x_avg * 1e-3) / 1e9 if x_avg else None},
+        "finalise_ms": fin_ms / args.steps,
+    }
+    dom = "fstage" if f_ms >= x_ms else "xstage"
+    achieved = stages[dom]["GBps"]
+    traffic = traffic_from_profiles(args.workload, args.precision)
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": (traffic or {}).get(dom),
+        "algorithmic_bytes_per_launch": stages[dom]["bytes_per_launch"],
+    }
+
+    h2d = None
+    if not args.no_h2d:
+        host = [b.cpu().numpy() for b in bufs]
+        kh = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                                precision=args.precision, autos=True)
+        kh.setInputData(host)
+        kh.setDelays(poly, offs)
+        kh.process()
+        n = 3
+        t = time.perf_counter()
+        for _ in range(n):
+            kh.process()
+        h2d = w.input_samples * n / (time.perf_counter() - t) / 1e9
+        kh.close()
+        del host
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, poly, offs, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (seeded signal model generated in HBM: common signal, quadratic delays, fringes, 2-bit quantised)",
+        "config": {
+            "workload": w.name,
+            "stations": w.nant, "pols": 2, "nchan": w.nchan, "fft": w.fftchannels,
+            "numffts": w.numffts, "bandwidth_hz": w.bw, "lo_hz": w.lo,
+            "input": "complex int8" if w.is_complex else "real 2-bit",
+            "precision": args.precision, "autos": True,
+            "input_samples_per_step": w.input_samples,
+        },
+        "roofline": roofline,
+        "stages": stages,
+        "gbps_input": value * (16 if w.is_complex else 2),
+        "pcie_inclusive_gsamples_per_s": h2d,
+        "cpu_baseline": cpu,
+    }
+    k.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

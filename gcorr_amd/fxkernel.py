@@ -1,0 +1,184 @@
+"""Python mirror of fxkernel's FxKernel over the gxcorr C-ABI.
+
+Same constructor arguments and methods as fxkernel/src/fxkernel.h:18-64
+(setInputData, setDelays, process, accumulate, saveVisibilities), plus
+access to the visibilities as a numpy array and the stage entry points used
+by tests and benchmarks.  Visibilities are [row][product][channel] complex64:
+cross baselines in fxkernel order (fxkernel.cpp:328-355), product p = 2l+m
+= X_j,l * conj(X_k,m), then (with autos=True) one row per station.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact_phase,
+            chunk_ffts):
+    c = _lib.Config()
+    _lib.load().gxc_config_init(ctypes.byref(c))
+    c.nant, c.nchan, c.numffts, c.nbit = int(nant), int(nchan), int(nfft), int(nbit)
+    c.npol = 2
+    c.is_complex = int(bool(is_complex))
+    c.lo, c.bw = float(lo), float(bw)
+    c.precision = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
+    c.autos = int(bool(autos))
+    c.exact_phase = int(bool(exact_phase))
+    c.chunk_ffts = int(chunk_ffts)
+    return c
+
+
+def station_bytes(nant, nchan, nfft, nbit=2, is_complex=False):
+    """allocDataHost's per-station size (fxkernel/src/common.cpp:152)."""
+    c = _config(nant, nchan, nfft, nbit, 0.0, 1.0, is_complex, 0, False, False, 0)
+    return int(_lib.load().gxc_station_bytes(ctypes.byref(c)))
+
+
+def num_rows(nant, autos=False):
+    return nant * (nant - 1) // 2 + (nant if autos else 0)
+
+
+class FxKernel:
+    """FxKernel(nant, nchan, nfft, nbit, lo, bw, is_complex) on the MI355X path."""
+
+    def __init__(self, nant, nchan, nfft, nbit, lo, bw, is_complex=False, *, precision="fp32",
+                 autos=False, exact_phase=False, chunk_ffts=0):
+        self._L = _lib.load()
+        self.cfg = _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos,
+                           exact_phase, chunk_ffts)
+        self.nant, self.nchan, self.numffts = int(nant), int(nchan), int(nfft)
+        self.nrows = int(self._L.gxc_num_rows(ctypes.byref(self.cfg)))
+        self.station_bytes = int(self._L.gxc_station_bytes(ctypes.byref(self.cfg)))
+        h = ctypes.c_void_p()
+        check(self._L.gxc_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = None
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.gxc_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- fxkernel interface ---------------------------------------------------
+    def setInputData(self, idata):
+        """Host buffers, one per station (fxkernel.cpp:242-245).  Kept alive here."""
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in idata]
+        if len(arrs) != self.nant:
+            raise ValueError("need %d station buffers" % self.nant)
+        for a in arrs:
+            if a.size < self.station_bytes:
+                raise ValueError("station buffer has %d bytes, need %d" % (a.size, self.station_bytes))
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * self.nant)(
+            *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) for a in arrs])
+        check(self._L.gxc_set_input(self._h, ptrs), self._h)
+        self._keep = (arrs, ptrs)
+
+    def setInputDevice(self, dev_ptrs, keepalive=None):
+        """Device-resident station buffers (integers, e.g. tensor.data_ptr())."""
+        if len(dev_ptrs) != self.nant:
+            raise ValueError("need %d station buffers" % self.nant)
+        ptrs = (ctypes.c_void_p * self.nant)(*[int(p) for p in dev_ptrs])
+        check(self._L.gxc_set_input_device(self._h, ptrs), self._h)
+        self._keep = (keepalive, ptrs)
+
+    def setDelays(self, d, f):
+        """d: [nant][3] polynomial (s, t in FFTs), f: [nant] file offsets (s)."""
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(self.nant, 3)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.nant)
+        rows = (ctypes.POINTER(ctypes.c_double) * self.nant)(
+            *[d[a].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in range(self.nant)])
+        check(self._L.gxc_set_delays(self._h, rows, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              self._h)
+
+    def process(self):
+        check(self._L.gxc_process(self._h), self._h)
+
+    def process_async(self):
+        check(self._L.gxc_process_async(self._h), self._h)
+
+    def synchronize(self):
+        check(self._L.gxc_synchronize(self._h), self._h)
+
+    def accumulate(self, odata):
+        """odata: complex64 array [rows][4][nchan], updated in place (fxkernel.cpp:414-423)."""
+        if odata.dtype != np.complex64 or odata.shape != (self.nrows, 4, self.nchan) \
+                or not odata.flags.c_contiguous:
+            raise ValueError("odata must be C-contiguous complex64 of shape %s"
+                             % ((self.nrows, 4, self.nchan),))
+        base = odata.ctypes.data
+        row_t = ctypes.POINTER(_lib.CF32) * 4
+        rows = (ctypes.POINTER(ctypes.POINTER(_lib.CF32)) * self.nrows)()
+        keep = []
+        for r in range(self.nrows):
+            prods = row_t(*[ctypes.cast(base + ((r * 4 + p) * self.nchan) * 8,
+                                        ctypes.POINTER(_lib.CF32)) for p in range(4)])
+            keep.append(prods)
+            rows[r] = ctypes.cast(prods, ctypes.POINTER(ctypes.POINTER(_lib.CF32)))
+        check(self._L.gxc_accumulate(self._h, ctypes.cast(rows, ctypes.c_void_p)), self._h)
+
+    def saveVisibilities(self, path):
+        check(self._L.gxc_save_visibilities(self._h, str(path).encode()), self._h)
+
+    # -- extras -----------------------------------------------------------------
+    def visibilities(self):
+        out = np.empty((self.nrows, 4, self.nchan), np.complex64)
+        check(self._L.gxc_get_visibilities(self._h, out.ctypes.data_as(ctypes.c_void_p), out.size),
+              self._h)
+        return out
+
+    def row_counts(self):
+        out = np.empty(self.nrows, np.int32)
+        check(self._L.gxc_get_row_counts(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         out.size), self._h)
+        return out
+
+    def fstage(self, t0, nfft):
+        """Spectra [nfft][2*nant][nchan] complex64 and validity [nant][nfft] of FFTs t0..t0+nfft."""
+        spec = np.empty((nfft, 2 * self.nant, self.nchan), np.complex64)
+        valid = np.empty((self.nant, nfft), np.uint8)
+        check(self._L.gxc_fstage(self._h, int(t0), int(nfft), spec.ctypes.data_as(ctypes.c_void_p),
+                                 valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), self._h)
+        return spec, valid.astype(bool)
+
+    def set_profiling(self, on=True):
+        check(self._L.gxc_set_profiling(self._h, int(bool(on))), self._h)
+
+    def timings(self):
+        t = _lib.Timings()
+        check(self._L.gxc_get_timings(self._h, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in _lib.Timings._fields_}
+
+    # snake_case aliases
+    set_input_data = setInputData
+    set_input_device = setInputDevice
+    set_delays = setDelays
+    save_visibilities = saveVisibilities
+
+
+def xengine(spectra, nant, autos=False, precision="fp32"):
+    """X-stage alone over spectra [nfft][2*nant][nchan]; result / nfft (gcorr CrossCorr semantics)."""
+    L = _lib.load()
+    spectra = np.ascontiguousarray(spectra, np.complex64)
+    nfft, ninp, nchan = spectra.shape
+    if ninp != 2 * nant:
+        raise ValueError("spectra must have 2*nant inputs")
+    out = np.empty((num_rows(nant, autos), 4, nchan), np.complex64)
+    prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
+    check(L.gxc_xengine(spectra.ctypes.data_as(ctypes.c_void_p), int(nant), int(nchan), int(nfft),
+                        int(bool(autos)), prec, out.ctypes.data_as(ctypes.c_void_p)))
+    return out

@@ -1,0 +1,202 @@
+/*
+ * gxcorr.h -- C-ABI of the MI355X FX-correlation hot path.
+ *
+ * Drop-in for the per-subintegration processing interface of fxkernel
+ * (fxkernel/src/fxkernel.h:18-183) and for the GPU call sequence of gcorr
+ * (gcorr/src/testgpukernel.cu:462-534).  All arguments are plain pointers,
+ * sizes and PODs; no C++ or framework types cross this boundary.
+ *
+ * Data layouts (identical to the reference):
+ *   input       one packed buffer per station, (numffts+1)*fftchannels time
+ *               samples (fxkernel/src/common.cpp:152).  Real 2-bit: 4 samples
+ *               per byte, LSB first, [A_t, B_t, A_t+1, B_t+1]
+ *               (fxkernel/src/fxkernel.cpp:443-476).  Complex 8-bit
+ *               (extension): [Are, Aim, Bre, Bim] int8 per time sample.
+ *   delays      per station a quadratic d(t)=d[0]*t^2+d[1]*t+d[2] seconds,
+ *               t in FFT intervals, plus a file start offset in seconds
+ *               (fxkernel/src/fxkernel.h:42-47, common.cpp:219-223).
+ *   visibilities [row][product][channel] complex float, rows 0..nbl-1 the
+ *               cross baselines (0,1),(0,2)..(0,N-1),(1,2).. and product
+ *               p = 2*l+m = X_j,l * conj(X_k,m) (fxkernel.cpp:327-356);
+ *               when autos are enabled rows nbl..nbl+nant-1 hold station
+ *               autocorrelations with the same product order.
+ *
+ * Errors are returned as gxc_status; gxc_last_error() gives text.
+ * A handle owns one HIP stream and its device buffers; handles are
+ * independent and may be used from different threads; one handle must not
+ * be used from two threads at once (fxkernel objects are one per thread,
+ * fxkernel/src/bench_fxkernel.cpp:217-263).
+ */
+#ifndef GXCORR_H
+#define GXCORR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GXC_ABI_VERSION 1
+
+/* Layout-compatible with fxkernel's cf32 (Ipp32fc, fxkernel/src/vectordefs.h:16). */
+typedef struct {
+  float re;
+  float im;
+} gxc_cf32;
+
+typedef enum {
+  GXC_OK = 0,
+  GXC_ERR_INVALID_ARG = 1, /* bad config or argument (fxkernel exits, fxkernel.cpp:47-63) */
+  GXC_ERR_UNSUPPORTED = 2, /* valid but not supported by this build */
+  GXC_ERR_NO_DEVICE = 3,   /* no usable gfx950 GPU */
+  GXC_ERR_HIP = 4,         /* a HIP runtime call failed */
+  GXC_ERR_OUT_OF_MEMORY = 5,
+  GXC_ERR_STATE = 6,       /* input or delays not set before process */
+  GXC_ERR_IO = 7           /* file could not be read or written */
+} gxc_status;
+
+typedef enum {
+  GXC_PREC_FP32 = 0, /* drop-in: FP32 spectra, FP32 X-engine */
+  GXC_PREC_FP16 = 1, /* FP16 spectra (per-channel RMS scaled), MFMA X-engine */
+  GXC_PREC_BF16 = 2, /* BF16 spectra, MFMA X-engine */
+  GXC_PREC_FP8 = 3   /* FP8 e4m3 spectra with per-channel/per-station scales */
+} gxc_precision;
+
+/* Correlator configuration: the FxKernel constructor arguments
+ * (fxkernel/src/fxkernel.h:29) plus the .conf keywords NPOL
+ * (fxkernel/src/common.cpp:230-231) and this build's extensions. */
+typedef struct {
+  int nant;        /* NANT */
+  int nchan;       /* NCHAN, power of two */
+  int numffts;     /* NUMFFTS per subintegration */
+  int nbit;        /* NBIT: 2 (real) or 8 (complex) */
+  int npol;        /* NPOL: must be 2 (fxkernel.cpp:77-81 assumes dual pol) */
+  int is_complex;  /* COMPLEX */
+  double lo;       /* LO, Hz */
+  double bw;       /* BANDWIDTH, Hz */
+  int precision;   /* gxc_precision; GXC_PREC_FP32 is the drop-in */
+  int autos;       /* 1: append autocorrelation rows */
+  int exact_phase; /* 0: fxkernel's stride fringe-phase law (default);
+                      1: exact per-sample phase */
+  int chunk_ffts;  /* FFTs per device pass; 0 = choose automatically */
+  int reserved[6];
+} gxc_config;
+
+typedef struct gxc_handle gxc_handle;
+
+/* Per-stage device time of the most recent gxc_process, milliseconds,
+ * measured with HIP events on the handle's stream (profiling enabled). */
+typedef struct {
+  double h2d_ms;       /* host-to-device input copy (0 for device input) */
+  double fstage_ms;    /* unpack+rotate+FFT+frac (all chunks) */
+  double xstage_ms;    /* cross-multiply-accumulate (all chunks) */
+  double finalise_ms;  /* count + normalise */
+  double total_ms;     /* first to last event */
+  int fstage_launches; /* number of F-stage kernel launches */
+  int xstage_launches;
+} gxc_timings;
+
+/* Fill cfg with defaults: nbit 2, npol 2, real, FP32, no autos, stride law. */
+void gxc_config_init(gxc_config *cfg);
+
+/* FxKernel::FxKernel (fxkernel/src/fxkernel.cpp:30-180) */
+gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out);
+
+/* FxKernel::~FxKernel (fxkernel/src/fxkernel.cpp:182-240) */
+void gxc_destroy(gxc_handle *h);
+
+/* Text of the last error on this handle (h may be NULL: last error of the
+ * calling thread, e.g. from a failed gxc_create). */
+const char *gxc_last_error(const gxc_handle *h);
+const char *gxc_status_string(gxc_status s);
+
+/* FxKernel::setInputData (fxkernel/src/fxkernel.cpp:242-245): host buffers,
+ * one per station, each gxc_station_bytes() long.  Pointers are stored, not
+ * copied; they must stay valid until gxc_process returns. */
+gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station);
+
+/* As gxc_set_input but the buffers already live in device memory (HBM) of
+ * the current device: no copy is made (gcorr keeps packed data on the device,
+ * gcorr/src/testgpukernel.cu:468-470). */
+gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station);
+
+/* FxKernel::setDelays (fxkernel/src/fxkernel.cpp:247-251): poly3[a] points
+ * to 3 doubles, fileoffsets[a] seconds.  Values are copied at this call. */
+gxc_status gxc_set_delays(gxc_handle *h, const double *const *poly3,
+                          const double *fileoffsets);
+
+/* FxKernel::process (fxkernel/src/fxkernel.cpp:253-368): one subintegration.
+ * Synchronous: returns when the visibilities are complete on the device. */
+gxc_status gxc_process(gxc_handle *h);
+
+/* As gxc_process but only enqueues the work on the handle's stream. */
+gxc_status gxc_process_async(gxc_handle *h);
+
+/* Wait for work enqueued on the handle's stream. */
+gxc_status gxc_synchronize(gxc_handle *h);
+
+/* FxKernel::accumulate (fxkernel/src/fxkernel.cpp:414-423):
+ * odata[row][product][channel] += visibilities, rows 0..gxc_num_rows-1. */
+gxc_status gxc_accumulate(gxc_handle *h, gxc_cf32 *const *const *odata);
+
+/* Copy the visibilities [rows][4][nchan] to host memory (nelem >= total). */
+gxc_status gxc_get_visibilities(gxc_handle *h, gxc_cf32 *out, size_t nelem);
+
+/* Number of FFTs accumulated per row (fxkernel.cpp:352, baselineCount). */
+gxc_status gxc_get_row_counts(gxc_handle *h, int *counts, size_t n);
+
+/* FxKernel::saveVisibilities (fxkernel/src/fxkernel.cpp:370-412): same text
+ * format, one line per channel. */
+gxc_status gxc_save_visibilities(gxc_handle *h, const char *path);
+
+/* Device pointer of the visibilities [rows][4][nchan] cf32 (stream-ordered). */
+const void *gxc_visibilities_device(const gxc_handle *h);
+
+/* Bytes per station buffer, fxkernel/src/common.cpp:152 (numffts+1 FFTs). */
+size_t gxc_station_bytes(const gxc_config *cfg);
+
+/* Visibility rows: nant*(nant-1)/2 cross baselines (+ nant autos). */
+int gxc_num_rows(const gxc_config *cfg);
+
+/* Stage entry points used by tests and benchmarks; each mirrors a stage of
+ * gcorr's kernel sequence (gcorr/src/gxkernel.h:69-90).
+ *
+ * gxc_fstage: F-stage for FFTs [t0, t0+nfft) of the handle's input and
+ * delays.  spectra: host [nfft][nant*2][nchan] cf32 (may be NULL);
+ * valid: host [nant][nfft] (may be NULL).  For GXC_PREC_FP32 only. */
+gxc_status gxc_fstage(gxc_handle *h, int t0, int nfft, gxc_cf32 *spectra,
+                      uint8_t *valid);
+
+/* gxc_xengine: X-stage alone over host spectra [nfft][ninputs][nchan]
+ * (ninputs = 2*nant), all FFTs valid, result divided by nfft as in
+ * gcorr's CrossCorr+finaliseAccum (gcorr/src/gxkernel.cu:353-424) and
+ * written to out [rows][4][nchan] (rows per `autos`).  The integer
+ * known-answer test pattern of gcorr/test/unit.cc:73-159 runs through it. */
+gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft,
+                       int autos, int precision, gxc_cf32 *out);
+
+/* Per-stage timing (see gxc_timings).  Enabled by gxc_set_profiling. */
+gxc_status gxc_set_profiling(gxc_handle *h, int enable);
+gxc_status gxc_get_timings(const gxc_handle *h, gxc_timings *t);
+
+/* .conf reader, fxkernel/src/common.cpp:190-250 (same keywords, same
+ * defaults, unknown keywords reported and skipped).  The strings and arrays
+ * are owned by the struct and released by gxc_conffile_free. */
+typedef struct {
+  gxc_config cfg;
+  int nant_read;     /* antenna lines actually read */
+  char **names;      /* [nant] */
+  char **files;      /* [nant] */
+  double *delays;    /* [nant][3] */
+  double *offsets;   /* [nant] */
+} gxc_conffile;
+
+gxc_status gxc_parse_config(const char *path, gxc_conffile *out);
+void gxc_conffile_free(gxc_conffile *cf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GXCORR_H */

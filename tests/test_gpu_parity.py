@@ -1,0 +1,265 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle.
+
+FP32 visibilities must match the fxkernel restatement within 1e-5
+normalised error: eps = max over (row, product) of
+||V_gpu - V_ref||_2 / ||V_ref||_2 over channels.  Integer work (validity,
+row counts, the integer X known-answer test) must match exactly.
+"""
+import numpy as np
+import pytest
+
+import gcorr_amd
+from gcorr_amd import synth
+from oracle import fxref
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-5
+
+
+def eps(v, ref):
+    num = np.linalg.norm(v - ref, axis=-1)
+    den = np.linalg.norm(ref, axis=-1)
+    mask = den > 0
+    assert np.all(num[~mask] == 0)
+    return float((num[mask] / den[mask]).max()) if mask.any() else 0.0
+
+
+def random_case(nant, nchan, numffts, seed=1, lo=1.65e9, bw=64e6, is_complex=False):
+    w = synth.Workload("t", nant=nant, nchan=nchan, numffts=numffts, bw=bw, lo=lo,
+                       is_complex=is_complex, seed=seed)
+    rng = np.random.default_rng(seed)
+    data = [rng.integers(0, 256, w.station_bytes, dtype=np.uint8) for _ in range(nant)]
+    poly, offs = synth.make_delays(w)
+    return w, data, poly, offs
+
+
+def run_gpu(w, data, poly, offs, autos=True, **kw):
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                           autos=autos, **kw)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    v, c = k.visibilities(), k.row_counts()
+    k.close()
+    return v, c
+
+
+def run_oracle(w, data, poly, offs, autos=True, mode="f32", quirk=True):
+    cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=w.nbit,
+                            is_complex=w.is_complex, autos=autos, quirk=quirk)
+    return fxref.process(cfg, data, poly, offs, mode)
+
+
+# ------------------------------------------------------------------ F-stage
+
+@pytest.mark.parametrize("nchan,is_complex", [(32, False), (64, False), (128, False), (256, False),
+                                              (512, False), (1024, False), (2048, False),
+                                              (4096, False), (256, True), (1024, True),
+                                              (8192, True)])
+def test_fstage_matches_oracle(gpu, nchan, is_complex):
+    w, data, poly, offs = random_case(3, nchan, 40, seed=nchan + is_complex, is_complex=is_complex,
+                                      bw=128e6 if is_complex else 64e6)
+    # station 1: large net delay so FFT 0 is out of the buffer; station 2:
+    # negative net delay (odd sample offsets, last FFT out of the buffer).
+    Ts = w.sampletime
+    offs = offs.copy()
+    offs[1] = poly[1][2] - 5.3 * Ts
+    offs[2] = poly[2][2] + 3.6 * Ts
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    spec, valid = k.fstage(0, w.numffts)
+    k.close()
+    cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=w.nbit,
+                            is_complex=w.is_complex)
+    for a in range(w.nant):
+        for t in range(w.numffts):
+            ref, ok = fxref.station_spectrum(cfg, data[a], poly[a], offs[a], t)
+            assert bool(valid[a, t]) == ok, (a, t)
+            got = spec[t, 2 * a:2 * a + 2]
+            if not ok:
+                assert not np.any(got)
+                continue
+            rms = np.sqrt(np.mean(np.abs(ref) ** 2))
+            assert np.max(np.abs(got - ref)) / rms < 2e-5, (a, t)
+    assert not valid[1, 0] and not valid[2, w.numffts - 1]
+
+
+# ------------------------------------------------------------------ whole subintegration
+
+def test_process_c1_full_size(gpu):
+    """4 stations x 2 pols, 128 channels, 12 500 FFTs (0.1 s at 32 MS/s)."""
+    w = synth.WORKLOADS["c1"]
+    rng = np.random.default_rng(11)
+    data = [rng.integers(0, 256, w.station_bytes, dtype=np.uint8) for _ in range(w.nant)]
+    poly, offs = synth.make_delays(w)
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+
+
+def test_process_c2_shape_reduced(gpu):
+    """6 stations x 2 pols, 1024 channels, 2 000 FFTs of the 62 500."""
+    w = synth.WORKLOADS["c2"].scaled(2000)
+    rng = np.random.default_rng(12)
+    data = [rng.integers(0, 256, w.station_bytes, dtype=np.uint8) for _ in range(w.nant)]
+    poly, offs = synth.make_delays(w)
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+    # the GPU is at least as close to the FP64 oracle as the FP32 restatement is
+    r64, _ = run_oracle(w, data, poly, offs, mode="f64")
+    assert eps(v, r64) < EPS
+    assert eps(v, r64) <= 1.5 * eps(ref, r64) + 1e-7
+
+
+@pytest.mark.parametrize("nant,nchan,numffts", [(1, 256, 50), (2, 64, 300), (5, 512, 97),
+                                                (9, 128, 64), (16, 256, 24)])
+def test_process_shapes(gpu, nant, nchan, numffts):
+    w, data, poly, offs = random_case(nant, nchan, numffts, seed=nant * 100 + nchan)
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+
+
+def test_process_invalid_ffts_and_zero_count_rows(gpu):
+    w, data, poly, offs = random_case(4, 256, 60, seed=3)
+    Ts = w.sampletime
+    offs = offs.copy()
+    offs[0] = poly[0][2] - 7.4 * Ts     # first FFT invalid
+    offs[1] = poly[1][2] + 9.6 * Ts     # last FFT invalid
+    offs[3] = poly[3][2] + 1.0          # a second of offset: never valid
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert rc[0] == w.numffts - 2          # (0,1): both edges lost
+    assert rc[2] == 0 and not np.any(v[2])  # (0,3)
+    assert eps(v, ref) < EPS
+
+
+def test_process_exact_phase_law(gpu):
+    w, data, poly, offs = random_case(3, 1024, 80, seed=8)
+    v, c = run_gpu(w, data, poly, offs, exact_phase=True)
+    ref, rc = run_oracle(w, data, poly, offs, quirk=False)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+    stride_law, _ = run_oracle(w, data, poly, offs, quirk=True)
+    assert eps(stride_law, ref) > 10 * EPS  # the two laws are distinguishable here
+
+
+def test_process_fractional_lo(gpu):
+    w, data, poly, offs = random_case(3, 512, 50, seed=9, lo=1.65e9 + 0.375)
+    poly = poly.copy()
+    offs = offs.copy()
+    poly[:, 2] += 1.0   # whole seconds of delay exercise the (lo - int lo) term
+    offs += 1.0
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+
+
+@pytest.mark.parametrize("nchan", [256, 1024])
+def test_process_complex8(gpu, nchan):
+    w, data, poly, offs = random_case(4, nchan, 60, seed=21, bw=128e6, is_complex=True)
+    v, c = run_gpu(w, data, poly, offs)
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < EPS
+
+
+def test_chunked_equals_single_pass(gpu):
+    w, data, poly, offs = random_case(5, 256, 301, seed=4)
+    v1, c1 = run_gpu(w, data, poly, offs)
+    v2, c2 = run_gpu(w, data, poly, offs, chunk_ffts=37)
+    np.testing.assert_array_equal(c1, c2)
+    assert eps(v2, v1) < 1e-6
+    ref, _ = run_oracle(w, data, poly, offs)
+    assert eps(v2, ref) < EPS
+
+
+def test_process_is_deterministic_and_reusable(gpu):
+    w, data, poly, offs = random_case(4, 512, 120, seed=5)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    a = k.visibilities()
+    k.process()
+    b = k.visibilities()
+    np.testing.assert_array_equal(a, b)
+    # new delays on the same handle
+    offs2 = offs + 2.0 * w.sampletime
+    k.setDelays(poly, offs2)
+    k.process()
+    ref, _ = run_oracle(w, data, poly, offs2)
+    assert eps(k.visibilities(), ref) < EPS
+    k.close()
+
+
+# ------------------------------------------------------------------ X-engine KAT
+
+@pytest.mark.parametrize("nant,nchan,nfft", [(3, 256, 4), (5, 1024, 100), (3, 512, 3000),
+                                             (16, 128, 40)])
+def test_xengine_integer_kat(gpu, nant, nchan, nfft):
+    """gcorr/test/unit.cc:73-159: integer spectra in [-5, 5] make every sum
+    exact, so the GPU result equals the exact sum / nfft bit for bit."""
+    rng = np.random.default_rng(12335 + nfft)
+    X = (rng.integers(-5, 6, (nfft, 2 * nant, nchan))
+         + 1j * rng.integers(-5, 6, (nfft, 2 * nant, nchan))).astype(np.complex64)
+    got = gcorr_amd.xengine(X, nant, autos=True)
+    exact = fxref.xmac(X, nant, autos=True)  # sequential FP32, exact for these inputs
+    Xd = X.astype(np.complex128)
+    rows = [(j, k) for j in range(nant) for k in range(j + 1, nant)] + [(a, a) for a in range(nant)]
+    for r, (j, k) in enumerate(rows):
+        for l in range(2):
+            for m in range(2):
+                s = np.sum(Xd[:, 2 * j + l] * np.conj(Xd[:, 2 * k + m]), axis=0)
+                np.testing.assert_array_equal(exact[r, 2 * l + m], s)
+                expect = (s.real.astype(np.float32) / np.float32(nfft)) \
+                    + 1j * (s.imag.astype(np.float32) / np.float32(nfft))
+                np.testing.assert_array_equal(got[r, 2 * l + m], expect.astype(np.complex64))
+
+
+# ------------------------------------------------------------------ interface
+
+def test_accumulate_and_save(gpu, tmp_path):
+    w, data, poly, offs = random_case(3, 64, 30, seed=6)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    v = k.visibilities()
+    acc = np.ones((3, 4, 64), np.complex64)
+    k.accumulate(acc)
+    k.accumulate(acc)
+    np.testing.assert_allclose(acc, 1 + 2 * v, rtol=1e-6, atol=1e-3)
+    p = tmp_path / "vis.out"
+    k.saveVisibilities(str(p))
+    k.close()
+    rows = [ln.split() for ln in p.read_text().splitlines()]
+    assert len(rows) == 64 and all(len(r) == 2 + 3 * 4 * 4 for r in rows)
+    for c in (0, 17, 63):
+        r = rows[c]
+        assert int(r[0]) == c
+        assert float(r[1]) == pytest.approx((c + 0.5) / 64 * w.bw / 1e6, abs=1e-6)
+        for b in range(3):
+            for p_ in range(4):
+                re, im, amp, ph = (float(x) for x in r[2 + 4 * (4 * b + p_):6 + 4 * (4 * b + p_)])
+                z = v[b, p_, c]
+                assert re == pytest.approx(z.real, abs=1e-5 + 1e-6 * abs(z))
+                assert im == pytest.approx(z.imag, abs=1e-5 + 1e-6 * abs(z))
+                assert amp == pytest.approx(abs(z), abs=1e-5 + 1e-6 * abs(z))
+
+
+def test_errors_are_reported(gpu):
+    k = gcorr_amd.FxKernel(2, 64, 10, 2, 1.65e9, 16e6)
+    with pytest.raises(gcorr_amd.GxcError):
+        k.process()                    # no input, no delays
+    with pytest.raises(ValueError):
+        k.setInputData([np.zeros(10, np.uint8)] * 2)
+    k.close()
