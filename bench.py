@@ -40,6 +40,10 @@ def parse():
                     help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-h2d", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true", help="F and X on one stream")
+    ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (scales HBM counters)")
     return ap.parse_args()
 
 
@@ -106,8 +110,15 @@ def main():
     gen_s = time.time() - t
     print("generated %s in %.1f s" % (w.name, gen_s), file=sys.stderr, flush=True)
 
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
-                           precision=args.precision, autos=True)
+                           precision=args.precision, autos=True, overlap=not args.no_overlap,
+                           chunk_ffts=args.chunk_ffts)
     k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
     k.setDelays(poly, offs)
     for _ in range(args.warmup):
@@ -129,6 +140,15 @@ def main():
     t1 = time.perf_counter()
     k.set_profiling(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    # The timed region records HIP events around every launch, which runs the
+    # launches eagerly; the same steps replayed as one captured graph:
+    k.process()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        k.process()
+    torch.cuda.synchronize()
+    graph_ms_per_step = (time.perf_counter() - t) * 1e3 / args.steps
     value = w.input_samples / (ms_per_step * 1e-3) / 1e9
 
     # algorithmic bytes per launch: input bits + nchan-bin spectra (F);
@@ -149,6 +169,7 @@ def main():
         "xstage": {"ms_per_launch": x_avg, "launches_per_step": x_launch / args.steps,
                    "bytes_per_launch": x_bytes, "GBps": x_bytes / (x_avg * 1e-3) / 1e9 if x_avg else None},
         "finalise_ms": fin_ms / args.steps,
+        "graph_ms_per_step": graph_ms_per_step,
     }
     dom = "fstage" if f_ms >= x_ms else "xstage"
     achieved = stages[dom]["GBps"]
@@ -203,6 +224,7 @@ def main():
             "numffts": w.numffts, "bandwidth_hz": w.bw, "lo_hz": w.lo,
             "input": "complex int8" if w.is_complex else "real 2-bit",
             "precision": args.precision, "autos": True,
+            "overlap": not args.no_overlap, "chunk_ffts": args.chunk_ffts or "auto",
             "input_samples_per_step": w.input_samples,
         },
         "roofline": roofline,

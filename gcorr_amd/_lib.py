@@ -30,7 +30,8 @@ class Config(ctypes.Structure):
         ("nbit", ctypes.c_int), ("npol", ctypes.c_int), ("is_complex", ctypes.c_int),
         ("lo", ctypes.c_double), ("bw", ctypes.c_double),
         ("precision", ctypes.c_int), ("autos", ctypes.c_int), ("exact_phase", ctypes.c_int),
-        ("chunk_ffts", ctypes.c_int), ("reserved", ctypes.c_int * 6),
+        ("chunk_ffts", ctypes.c_int), ("overlap", ctypes.c_int), ("graph", ctypes.c_int),
+        ("reserved", ctypes.c_int * 4),
     ]
 
 

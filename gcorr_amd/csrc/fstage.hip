@@ -175,23 +175,34 @@ __device__ __forceinline__ bool split_offset(const FParams &p, int st, int t, in
   return !(off < 0 || off > p.maxoffset);
 }
 
+// LDS index of element i with one pad slot per 16 elements.  When a stride
+// is a multiple of 16 the padded address of i + k*stride is the padded
+// address of i plus k*stride*17/16, a compile-time offset.
+template <int STRIDE>
+__device__ __forceinline__ int pad_step(int k) {
+  return (STRIDE % 16 == 0) ? k * (STRIDE / 16) * 17 : -1;
+}
+
 template <int L, bool CPLX, int P>
 __device__ __forceinline__ void pass_from_lds(float2 (&va)[16], float2 (&vb)[16], const float2 *sa,
                                               const float2 *sb, int j, const float2 *__restrict__ tw) {
   using G = FG<L, CPLX>;
   constexpr int R = G::radix(P), NS = G::ns(P), NB = 16 / R, T = G::T;
+  constexpr int STR = L / R;
 #pragma unroll
   for (int bf = 0; bf < NB; bf++) {
     const int jj = j + bf * T;
+    const int b0 = padi(jj);
 #pragma unroll
     for (int k = 0; k < R; k++) {
-      va[bf * R + k] = sa[padi(jj + k * (L / R))];
-      vb[bf * R + k] = sb[padi(jj + k * (L / R))];
+      const int idx = (STR % 16 == 0) ? b0 + pad_step<STR>(k) : padi(jj + k * STR);
+      va[bf * R + k] = sa[idx];
+      vb[bf * R + k] = sb[idx];
     }
-    const int m = jj % NS;
+    const unsigned m = (unsigned)(jj % NS) * (unsigned)(L / (NS * R));
 #pragma unroll
     for (int k = 1; k < R; k++) {
-      const float2 w = tw[m * k * (L / (NS * R))];
+      const float2 w = tw[m * (unsigned)k];
       va[bf * R + k] = cmul(va[bf * R + k], w);
       vb[bf * R + k] = cmul(vb[bf * R + k], w);
     }
@@ -209,10 +220,14 @@ __device__ __forceinline__ void pass_to_lds(const float2 (&va)[16], const float2
   for (int bf = 0; bf < NB; bf++) {
     const int jj = j + bf * T;
     const int base = (jj / NS) * NS * R + (jj % NS);
+    const int b0 = padi(base);
 #pragma unroll
     for (int k = 0; k < R; k++) {
-      sa[padi(base + k * NS)] = va[bf * R + k];
-      sb[padi(base + k * NS)] = vb[bf * R + k];
+      // NS == 1: base = R*jj, so base + k stays inside one 16-block (R <= 16).
+      const int idx = (NS % 16 == 0) ? b0 + pad_step<NS>(k) : (NS == 1 && R == 16) ? b0 + k
+                                                                                   : padi(base + k * NS);
+      sa[idx] = va[bf * R + k];
+      sb[idx] = vb[bf * R + k];
     }
   }
 }
@@ -231,13 +246,34 @@ __device__ __forceinline__ void middle_passes(float2 (&va)[16], float2 (&vb)[16]
   }
 }
 
-template <int L, bool CPLX>
+__device__ __forceinline__ uint32_t pack_f16(float re, float im) {
+  const _Float16 a = (_Float16)re, b = (_Float16)im;
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(float re, float im) {
+  return bf16_bits(re) | (bf16_bits(im) << 16);
+}
+
+__device__ __forceinline__ uint16_t pack_fp8(float re, float im) {
+  // e4m3 with saturation to the largest finite value before conversion.
+  re = fminf(fmaxf(re, -448.f), 448.f);
+  im = fminf(fmaxf(im, -448.f), 448.f);
+  return (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false) & 0xffff);
+}
+
+template <int L, bool CPLX, int OUT>
 __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p) {
   using G = FG<L, CPLX>;
   constexpr int T = G::T, S = G::S, S2 = G::S2, NSTEPS = G::NSTEPS;
   __shared__ float2 sdata[G::IPW][2][G::PADL];
   __shared__ float2 stab[G::IPW][G::NTAB];
-  __shared__ float2 slut[CPLX ? 1 : 512];
+  __shared__ float2 slut[16];
 
   const int li = threadIdx.x / T;
   const int j = threadIdx.x % T;
@@ -245,18 +281,17 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
   const bool active = item < p.nant * p.nt;
   const int tl = active ? item / p.nant : 0;
   const int st = active ? item % p.nant : 0;
-  const int t = p.t0 + tl;
+  const int t = p.t0 + tl * p.tstride;
 
   if constexpr (!CPLX) {
-    // slut[2*byte + slot] = (pol A, pol B) levels of time slot `slot` of `byte`
-    // (fxkernel.cpp:15-28 LUT; byte layout [A_t, B_t, A_t+1, B_t+1]).
-    for (int e = threadIdx.x; e < 512; e += G::BLOCK) {
-      const int byte = e >> 1, slot = e & 1;
-      const int ca = (byte >> (4 * slot)) & 3, cb = (byte >> (4 * slot + 2)) & 3;
-      const float lv0 = -kHiMag, lv1 = -1.0f, lv2 = 1.0f, lv3 = kHiMag;
-      const float xa = ca == 0 ? lv0 : ca == 1 ? lv1 : ca == 2 ? lv2 : lv3;
-      const float xb = cb == 0 ? lv0 : cb == 1 ? lv1 : cb == 2 ? lv2 : lv3;
-      slut[e] = make_float2(xa, xb);
+    // slut[nibble] = (pol A, pol B) levels of one time slot: the byte layout
+    // is [A_t, B_t, A_t+1, B_t+1] (fxkernel.cpp:15-28,443-476), so a time
+    // slot is the nibble (byte >> 4*slot) & 15 with A in its low 2 bits.
+    // 16 float2 entries occupy 32 consecutive dwords, one per LDS bank:
+    // lanes reading different entries never share a bank.
+    if (threadIdx.x < 16) {
+      const float lv[4] = {-kHiMag, -1.0f, 1.0f, kHiMag};
+      slut[threadIdx.x] = make_float2(lv[threadIdx.x & 3], lv[threadIdx.x >> 2]);
     }
   }
 
@@ -310,20 +345,32 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
 
   // ---- pass 0: unpack + rotate from global memory, radix-16, no twiddles ----
   if (valid) {
-    const uint8_t *src = p.station[st];
+    // Station buffers are device (global) memory; saying so lets the loads
+    // below use the global rather than the flat address path.
+    typedef const __attribute__((address_space(1))) uint8_t gbyte;
+    gbyte *src = (gbyte *)p.station[st];
+    // Sample n = j + k*L/16 of the window is packed sample offset+n.  L/16
+    // is even, so every k shares the byte parity of offset+j: one base
+    // pointer, compile-time byte offsets k*L/32, one nibble shift.
+    const int s0 = offset + j;
+    gbyte *bsrc = src + (CPLX ? 4 * (size_t)s0 : (size_t)(s0 >> 1));
+    const int shift = (s0 & 1) * 4;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int n = j + k * (L / 16);
       const float2 r = cmul(sub[n % S2], step[n / S2]);  // complexrotator[n], fxkernel.cpp:553-558
-      const int sidx = offset + n;
       if constexpr (!CPLX) {
-        const float2 x = slut[2 * (int)src[sidx >> 1] + (sidx & 1)];
+        const float2 x = slut[(bsrc[k * (L / 32)] >> shift) & 15];
         va[k] = make_float2(x.x * r.x, x.x * r.y);
         vb[k] = make_float2(x.y * r.x, x.y * r.y);
       } else {
-        const char4 c = *reinterpret_cast<const char4 *>(src + 4 * (size_t)sidx);
-        va[k] = cmul(r, make_float2((float)c.x, (float)c.y));
-        vb[k] = cmul(r, make_float2((float)c.z, (float)c.w));
+        // [Are, Aim, Bre, Bim] int8, little-endian in one 32-bit word
+        const int wv = *reinterpret_cast<const __attribute__((address_space(1))) int *>(
+            bsrc + 4 * (size_t)k * (L / 16));
+        const float are = (float)(int8_t)(wv & 0xff), aim = (float)(int8_t)((wv >> 8) & 0xff);
+        const float bre = (float)(int8_t)((wv >> 16) & 0xff), bim = (float)(int8_t)(wv >> 24);
+        va[k] = cmul(r, make_float2(are, aim));
+        vb[k] = cmul(r, make_float2(bre, bim));
       }
     }
     dft<16>(va);
@@ -344,8 +391,7 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
     constexpr int PL = G::NPASS - 1;
     constexpr int R = G::radix(PL), NB = 16 / R;
     constexpr int KEEP = CPLX ? R : R / 2;
-    float2 *outA = p.spectra + ((size_t)tl * p.nant + st) * 2 * G::NCHAN;
-    float2 *outB = outA + G::NCHAN;
+    const size_t rowA = ((size_t)tl * p.nant + st) * 2;  // output row of pol A; pol B follows
 #pragma unroll
     for (int bf = 0; bf < NB; bf++) {
       const int jj = j + bf * T;
@@ -358,11 +404,35 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
           ya = cmul(fr, va[bf * R + k]);
           yb = cmul(fr, vb[bf * R + k]);
         }
-        outA[q] = ya;
-        outB[q] = yb;
+        if constexpr (OUT == OUT_F32) {
+          p.spectra[rowA * G::NCHAN + q] = ya;
+          p.spectra[(rowA + 1) * G::NCHAN + q] = yb;
+        } else if constexpr (OUT == OUT_POWER) {
+          // one row per sampled FFT; the scale kernel sums rows in a fixed
+          // order, so the scales do not depend on scheduling
+          p.power[rowA * G::NCHAN + q] = ya.x * ya.x + ya.y * ya.y;
+          p.power[(rowA + 1) * G::NCHAN + q] = yb.x * yb.x + yb.y * yb.y;
+        } else {
+          const float sa = p.scales[(size_t)(2 * st) * G::NCHAN + q];
+          const float sb = p.scales[(size_t)(2 * st + 1) * G::NCHAN + q];
+          if constexpr (OUT == OUT_FP8) {
+            uint16_t *o = static_cast<uint16_t *>(p.out);
+            o[rowA * G::NCHAN + q] = pack_fp8(ya.x * sa, ya.y * sa);
+            o[(rowA + 1) * G::NCHAN + q] = pack_fp8(yb.x * sb, yb.y * sb);
+          } else {
+            uint32_t *o = static_cast<uint32_t *>(p.out);
+            if constexpr (OUT == OUT_F16) {
+              o[rowA * G::NCHAN + q] = pack_f16(ya.x * sa, ya.y * sa);
+              o[(rowA + 1) * G::NCHAN + q] = pack_f16(yb.x * sb, yb.y * sb);
+            } else {
+              o[rowA * G::NCHAN + q] = pack_bf16(ya.x * sa, ya.y * sa);
+              o[(rowA + 1) * G::NCHAN + q] = pack_bf16(yb.x * sb, yb.y * sb);
+            }
+          }
+        }
       }
     }
-    if (j == 0) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
+    if (OUT != OUT_POWER && j == 0) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
   }
 }
 
@@ -373,7 +443,15 @@ hipError_t launch_one(const FParams &p, hipStream_t s) {
   const long long blocks = (items + G::IPW - 1) / G::IPW;
   if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((fstage_kernel<L, CPLX>), dim3((unsigned)blocks), dim3(G::BLOCK), 0, s, p);
+  const dim3 grid((unsigned)blocks), block(G::BLOCK);
+  switch (p.out_mode) {
+    case OUT_F32: hipLaunchKernelGGL((fstage_kernel<L, CPLX, OUT_F32>), grid, block, 0, s, p); break;
+    case OUT_F16: hipLaunchKernelGGL((fstage_kernel<L, CPLX, OUT_F16>), grid, block, 0, s, p); break;
+    case OUT_BF16: hipLaunchKernelGGL((fstage_kernel<L, CPLX, OUT_BF16>), grid, block, 0, s, p); break;
+    case OUT_FP8: hipLaunchKernelGGL((fstage_kernel<L, CPLX, OUT_FP8>), grid, block, 0, s, p); break;
+    case OUT_POWER: hipLaunchKernelGGL((fstage_kernel<L, CPLX, OUT_POWER>), grid, block, 0, s, p); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

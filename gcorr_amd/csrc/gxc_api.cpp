@@ -1,9 +1,20 @@
 // C-ABI of the FX-correlation hot path (include/gxcorr.h): FxKernel's
 // processing interface (fxkernel/src/fxkernel.h:18-183) over HIP kernels.
+//
+// One subintegration on the device:
+//   [H2D copy of host input]
+//   [reduced precision: F-stage power pass on 64 sampled FFTs -> scales]
+//   for each chunk of FFTs: F-stage -> spectra buffer (two buffers when
+//     overlapped), X-stage (MFMA) accumulating split partials
+//   row counts from the validity flags, finalise (sum partials, normalise)
+// With overlap on, F runs on the handle's stream and X on a second stream,
+// ordered by events per spectra buffer.  With graph on, everything after the
+// input copy is captured once as a hipGraph and replayed on later calls.
 #include "../../include/gxcorr.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,7 +24,7 @@
 #include <vector>
 
 #include "gxc_internal.hpp"
-#include "gxc_mixed.hpp"
+#include "gxc_xengine.hpp"
 
 using gxc::Geometry;
 
@@ -71,12 +82,15 @@ gxc_status derive(const gxc_config *c, Derived *d, std::string *why) {
     *why = "FFT length " + std::to_string(d->L) + " not supported (64..8192)";
     return GXC_ERR_UNSUPPORTED;
   }
-  const long long maxoff = (long long)(c->numffts - 1) * d->L;
+  if (gxc::xm_blocks(d->M) == 0) {
+    *why = "at most 32 stations (64 inputs) per handle";
+    return GXC_ERR_UNSUPPORTED;
+  }
   if ((long long)c->numffts * d->L + d->L > 0x7fffffffLL) {
     *why = "subintegration too long for 32-bit sample offsets";
     return GXC_ERR_UNSUPPORTED;
   }
-  d->maxoffset = (int)maxoff;
+  d->maxoffset = (c->numffts - 1) * d->L;  // fxkernel.cpp:272
   d->nbl = c->nant * (c->nant - 1) / 2;
   d->nrows = d->nbl + (c->autos ? c->nant : 0);
   // fxkernel.cpp:31,37-38
@@ -87,13 +101,19 @@ gxc_status derive(const gxc_config *c, Derived *d, std::string *why) {
   return GXC_OK;
 }
 
+int out_mode_of(int precision) {
+  static const int mode_of[4] = {gxc::OUT_F32, gxc::OUT_F16, gxc::OUT_BF16, gxc::OUT_FP8};
+  return mode_of[precision];
+}
+
 }  // namespace
 
 struct gxc_handle {
   gxc_config cfg{};
   Derived d;
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // F-stage, input copies, final join
+  hipStream_t xstream = nullptr;  // X-stage when overlapped
   std::string err;
 
   // input
@@ -112,19 +132,32 @@ struct gxc_handle {
   float2 *d_twiddle = nullptr;
   float *d_subfreq = nullptr, *d_stepfreq = nullptr;
 
-  // work buffers
-  int chunk = 0, sb = 1, nb = 0, ntiles = 0, splits = 1;
-  float2 *d_spectra = nullptr;
-  uint8_t *d_valid = nullptr;
-  float2 *d_partial = nullptr;
+  // spectra, X-stage and output
+  int out_mode = gxc::OUT_F32;
+  int chunk = 0, nbuf = 1, splits = 1;
+  void *d_spec[2] = {nullptr, nullptr};  // [chunk][M][nchan] packed complex
+  float *d_partial = nullptr;             // [splits][nchan][2][M][M]
+  float *d_scales = nullptr;              // [M][nchan] (reduced precision)
+  float *d_power = nullptr;               // [kPowerRows][M][nchan]
+  uint8_t *d_valid = nullptr;             // [nant][numffts]
   int *d_counts = nullptr;
   int2 *d_rowmap = nullptr;
   float2 *d_vis = nullptr;
-  gxc::MixedState *mixed = nullptr;
+
+  // ordering between the two streams
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_f[2] = {nullptr, nullptr}, ev_x[2] = {nullptr, nullptr};
+
+  // graph replay
+  hipGraphExec_t graph_exec = nullptr;
 
   // profiling
   bool profiling = false;
-  std::vector<hipEvent_t> events;
+  std::vector<hipEvent_t> tev;
+  struct Mark {
+    int kind;  // 0 F start, 1 F end, 2 X start, 3 X end, 4 finalise start, 5 finalise end, 6 H2D end
+    size_t ev;
+  };
+  std::vector<Mark> marks;
   gxc_timings timings{};
 };
 
@@ -142,6 +175,12 @@ gxc_status set_err(gxc_handle *h, gxc_status s, const std::string &msg) {
     if (e_ != hipSuccess)                                                                    \
       return set_err((h), e_ == hipErrorOutOfMemory ? GXC_ERR_OUT_OF_MEMORY : GXC_ERR_HIP,   \
                      std::string(#call) + ": " + hipGetErrorString(e_));                     \
+  } while (0)
+
+#define GXC_TRY(expr)            \
+  do {                           \
+    gxc_status s_ = (expr);      \
+    if (s_ != GXC_OK) return s_; \
   } while (0)
 
 gxc_status check_device(gxc_handle *h, int *dev) {
@@ -169,12 +208,6 @@ gxc_status dmalloc(gxc_handle *h, T **p, size_t nelem) {
   return GXC_OK;
 }
 
-#define GXC_TRY(expr)            \
-  do {                           \
-    gxc_status s_ = (expr);      \
-    if (s_ != GXC_OK) return s_; \
-  } while (0)
-
 // Rows: cross baselines in fxkernel order (fxkernel.cpp:328-355), then autos.
 std::vector<int2> make_rowmap(int nant, bool autos) {
   std::vector<int2> rm;
@@ -183,16 +216,6 @@ std::vector<int2> make_rowmap(int nant, bool autos) {
   if (autos)
     for (int a = 0; a < nant; a++) rm.push_back(make_int2(a, a));
   return rm;
-}
-
-int choose_splits(int nchan, int ntiles, int nt) {
-  const long long waves = (long long)((nchan + 63) / 64) * ntiles;
-  long long s = (4096 + waves - 1) / waves;
-  const long long maxs = nt / 16 > 0 ? nt / 16 : 1;
-  if (s > maxs) s = maxs;
-  if (s > 512) s = 512;
-  if (s < 1) s = 1;
-  return (int)s;
 }
 
 gxc_status upload_tables(gxc_handle *h) {
@@ -217,37 +240,34 @@ gxc_status upload_tables(gxc_handle *h) {
   return GXC_OK;
 }
 
-gxc_status ensure_spectra(gxc_handle *h, int nt) {
-  const size_t per_fft = (size_t)h->d.M * h->cfg.nchan;
-  if (h->d_spectra && nt <= h->chunk) return GXC_OK;
-  if (h->d_spectra) hipFree(h->d_spectra);
-  h->d_spectra = nullptr;
-  GXC_TRY(dmalloc(h, &h->d_spectra, per_fft * (size_t)nt));
-  h->chunk = nt;
-  return GXC_OK;
-}
-
-void record(gxc_handle *h, size_t idx) {
+void mark(gxc_handle *h, int kind, hipStream_t s) {
   if (!h->profiling) return;
-  while (h->events.size() <= idx) {
+  const size_t idx = h->marks.size();
+  while (h->tev.size() <= idx) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    h->events.push_back(e);
+    h->tev.push_back(e);
   }
-  hipEventRecord(h->events[idx], h->stream);
+  hipEventRecord(h->tev[idx], s);
+  h->marks.push_back({kind, idx});
 }
 
-gxc_status run_fstage(gxc_handle *h, int t0, int nt) {
+gxc_status run_fstage(gxc_handle *h, int t0, int nt, int mode, void *out, int tstride, hipStream_t s) {
   gxc::FParams p{};
   p.station = h->d_station_ptrs;
   p.delays = h->d_delays;
   p.subchanfreq = h->d_subfreq;
   p.stepchanfreq = h->d_stepfreq;
   p.twiddle = h->d_twiddle;
-  p.spectra = h->d_spectra;
+  p.spectra = static_cast<float2 *>(out);
+  p.out = out;
+  p.scales = h->d_scales;
+  p.power = h->d_power;
   p.valid = h->d_valid;
+  p.out_mode = mode;
   p.t0 = t0;
   p.nt = nt;
+  p.tstride = tstride;
   p.nant = h->cfg.nant;
   p.numffts = h->cfg.numffts;
   p.maxoffset = h->d.maxoffset;
@@ -257,105 +277,170 @@ gxc_status run_fstage(gxc_handle *h, int t0, int nt) {
   p.lofrac = lo - std::trunc(lo);
   p.fraclo = (lo - std::trunc(lo)) > 0.00000000001 ? 1 : 0;  // TINY, fxkernel.cpp:66-71
   p.sampletime = h->d.sampletime;
-  GXC_HIP(h, gxc::launch_fstage(p, h->d.L, h->cfg.is_complex != 0, h->stream));
+  GXC_HIP(h, gxc::launch_fstage(p, h->d.L, h->cfg.is_complex != 0, s));
   return GXC_OK;
 }
 
-gxc_status run_xstage(gxc_handle *h, int nt, int accumulate) {
-  gxc::XParams x{};
-  x.spectra = h->d_spectra;
+gxc_status run_xstage(gxc_handle *h, const void *spec, int nt, int accumulate, hipStream_t s) {
+  gxc::XMParams x{};
+  x.spectra = spec;
   x.partial = h->d_partial;
   x.nt = nt;
-  x.nant = h->cfg.nant;
+  x.M = h->d.M;
   x.nchan = h->cfg.nchan;
-  x.nb = h->nb;
-  x.ntiles = h->ntiles;
-  x.tile0 = 0;
   x.splits = h->splits;
   x.accumulate = accumulate;
-  GXC_HIP(h, gxc::launch_xstage(x, h->sb, h->stream));
+  GXC_HIP(h, gxc::launch_xmfma(x, h->out_mode, s));
   return GXC_OK;
 }
 
-gxc_status run_finalise(gxc_handle *h) {
+gxc_status run_finalise(gxc_handle *h, hipStream_t s) {
   gxc::CountParams cp{};
   cp.valid = h->d_valid;
   cp.rowmap = h->d_rowmap;
   cp.counts = h->d_counts;
   cp.numffts = h->cfg.numffts;
   cp.nrows = h->d.nrows;
-  GXC_HIP(h, gxc::launch_counts(cp, h->stream));
-  gxc::FinParams f{};
+  GXC_HIP(h, gxc::launch_counts(cp, s));
+  gxc::FinMParams f{};
   f.partial = h->d_partial;
+  f.scales = h->out_mode == gxc::OUT_F32 ? nullptr : h->d_scales;
   f.rowmap = h->d_rowmap;
   f.counts = h->d_counts;
   f.vis = h->d_vis;
-  f.nrows = h->d.nrows;
+  f.M = h->d.M;
   f.nchan = h->cfg.nchan;
-  f.nb = h->nb;
-  f.ntiles = h->ntiles;
+  f.nrows = h->d.nrows;
   f.splits = h->splits;
-  f.sb = h->sb;
-  GXC_HIP(h, gxc::launch_finalise(f, h->stream));
+  GXC_HIP(h, gxc::launch_finalise_mixed(f, s));
+  return GXC_OK;
+}
+
+// Device work of one subintegration once the input is resident.
+gxc_status enqueue_device_work(gxc_handle *h) {
+  const gxc_config &c = h->cfg;
+  const bool overlap = h->nbuf == 2;
+  hipStream_t fs = h->stream, xs = overlap ? h->xstream : h->stream;
+  if (h->out_mode != gxc::OUT_F32) {
+    // Per-(input, channel) scales from |X|^2 of FFTs sampled across the subint.
+    const int npow = std::min(c.numffts, gxc::kPowerRows);
+    GXC_TRY(run_fstage(h, 0, npow, gxc::OUT_POWER, nullptr, c.numffts / npow, fs));
+    GXC_HIP(h, gxc::launch_scales(h->d_scales, h->d_power, h->d.M * c.nchan, npow,
+                                  gxc::xm_target_rms(h->out_mode), fs));
+  }
+  if (overlap) {
+    GXC_HIP(h, hipEventRecord(h->ev_fork, fs));
+    GXC_HIP(h, hipStreamWaitEvent(xs, h->ev_fork, 0));
+  }
+  int nchunks = 0;
+  for (int t0 = 0; t0 < c.numffts; t0 += h->chunk, nchunks++) {
+    const int nt = std::min(h->chunk, c.numffts - t0);
+    const int b = overlap ? (nchunks & 1) : 0;
+    if (overlap && nchunks >= 2) GXC_HIP(h, hipStreamWaitEvent(fs, h->ev_x[b], 0));
+    mark(h, 0, fs);
+    GXC_TRY(run_fstage(h, t0, nt, h->out_mode, h->d_spec[b], 1, fs));
+    mark(h, 1, fs);
+    if (overlap) {
+      GXC_HIP(h, hipEventRecord(h->ev_f[b], fs));
+      GXC_HIP(h, hipStreamWaitEvent(xs, h->ev_f[b], 0));
+    }
+    mark(h, 2, xs);
+    GXC_TRY(run_xstage(h, h->d_spec[b], nt, t0 > 0 ? 1 : 0, xs));
+    mark(h, 3, xs);
+    if (overlap) GXC_HIP(h, hipEventRecord(h->ev_x[b], xs));
+  }
+  mark(h, 4, xs);
+  GXC_TRY(run_finalise(h, xs));
+  mark(h, 5, xs);
+  if (overlap) {
+    GXC_HIP(h, hipEventRecord(h->ev_join, xs));
+    GXC_HIP(h, hipStreamWaitEvent(fs, h->ev_join, 0));
+  }
+  h->timings.fstage_launches = nchunks;
+  h->timings.xstage_launches = nchunks;
+  return GXC_OK;
+}
+
+gxc_status ensure_owned_input(gxc_handle *h) {
+  if (h->d_input_owned) return GXC_OK;
+  const int nant = h->cfg.nant;
+  GXC_TRY(dmalloc(h, &h->d_input_owned, h->d.station_bytes * (size_t)nant));
+  std::vector<const uint8_t *> ptrs(nant);
+  for (int a = 0; a < nant; a++) ptrs[a] = h->d_input_owned + (size_t)a * h->d.station_bytes;
+  GXC_HIP(h, hipMemcpy(h->d_station_ptrs, ptrs.data(), nant * sizeof(void *), hipMemcpyHostToDevice));
+  return GXC_OK;
+}
+
+gxc_status enqueue_input_copy(gxc_handle *h) {
+  if (h->input_device) return GXC_OK;
+  GXC_TRY(ensure_owned_input(h));
+  for (int a = 0; a < h->cfg.nant; a++)
+    GXC_HIP(h, hipMemcpyAsync(h->d_input_owned + (size_t)a * h->d.station_bytes, h->host_in[a],
+                              h->d.station_bytes, hipMemcpyHostToDevice, h->stream));
   return GXC_OK;
 }
 
 gxc_status enqueue_process(gxc_handle *h) {
   if (!h->input_set) return set_err(h, GXC_ERR_STATE, "input data not set");
   if (!h->delays_set) return set_err(h, GXC_ERR_STATE, "delays not set");
-  const gxc_config &c = h->cfg;
-  size_t ev = 0;
-  record(h, ev++);  // 0: start
-  if (!h->input_device) {
-    if (!h->d_input_owned) {
-      GXC_TRY(dmalloc(h, &h->d_input_owned, h->d.station_bytes * (size_t)c.nant));
-      std::vector<const uint8_t *> ptrs(c.nant);
-      for (int a = 0; a < c.nant; a++) ptrs[a] = h->d_input_owned + (size_t)a * h->d.station_bytes;
-      GXC_HIP(h, hipMemcpy(h->d_station_ptrs, ptrs.data(), c.nant * sizeof(void *),
-                           hipMemcpyHostToDevice));
+  h->marks.clear();
+  mark(h, 7, h->stream);
+  GXC_TRY(enqueue_input_copy(h));
+  mark(h, 6, h->stream);
+  const bool use_graph = h->cfg.graph && !h->profiling;
+  if (!use_graph) return enqueue_device_work(h);
+  if (!h->graph_exec) {
+    hipGraph_t graph = nullptr;
+    GXC_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    gxc_status s = enqueue_device_work(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (s != GXC_OK) {
+      if (graph) hipGraphDestroy(graph);
+      return s;
     }
-    for (int a = 0; a < c.nant; a++)
-      GXC_HIP(h, hipMemcpyAsync(h->d_input_owned + (size_t)a * h->d.station_bytes, h->host_in[a],
-                                h->d.station_bytes, hipMemcpyHostToDevice, h->stream));
+    if (e != hipSuccess) return set_err(h, GXC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      h->graph_exec = nullptr;
+      return set_err(h, GXC_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
   }
-  record(h, ev++);  // 1: input resident
-  if (c.precision != GXC_PREC_FP32)
-    return set_err(h, GXC_ERR_UNSUPPORTED, "mixed-precision process not available in this build");
-  int nchunks = 0;
-  for (int t0 = 0; t0 < c.numffts; t0 += h->chunk, nchunks++) {
-    const int nt = std::min(h->chunk, c.numffts - t0);
-    record(h, ev++);
-    GXC_TRY(run_fstage(h, t0, nt));
-    record(h, ev++);
-    GXC_TRY(run_xstage(h, nt, t0 > 0 ? 1 : 0));
-    record(h, ev++);
-  }
-  GXC_TRY(run_finalise(h));
-  record(h, ev++);
-  h->timings.fstage_launches = nchunks;
-  h->timings.xstage_launches = nchunks;
+  GXC_HIP(h, hipGraphLaunch(h->graph_exec, h->stream));
   return GXC_OK;
 }
 
 float elapsed(gxc_handle *h, size_t a, size_t b) {
   float ms = 0.f;
-  if (b < h->events.size() && hipEventElapsedTime(&ms, h->events[a], h->events[b]) == hipSuccess)
+  if (a < h->tev.size() && b < h->tev.size() && hipEventElapsedTime(&ms, h->tev[a], h->tev[b]) == hipSuccess)
     return ms;
   return 0.f;
 }
 
 void collect_timings(gxc_handle *h) {
-  if (!h->profiling || h->events.size() < 3) return;
+  if (!h->profiling || h->marks.empty()) return;
   gxc_timings &t = h->timings;
-  const int n = t.fstage_launches;
-  t.h2d_ms = elapsed(h, 0, 1);
-  t.fstage_ms = t.xstage_ms = 0;
-  for (int k = 0; k < n; k++) {
-    t.fstage_ms += elapsed(h, 2 + 3 * k, 3 + 3 * k);
-    t.xstage_ms += elapsed(h, 3 + 3 * k, 4 + 3 * k);
+  t.h2d_ms = t.fstage_ms = t.xstage_ms = t.finalise_ms = t.total_ms = 0;
+  size_t first = h->marks.front().ev, last = h->marks.back().ev;
+  size_t open[8] = {0};
+  for (const auto &m : h->marks) {
+    switch (m.kind) {
+      case 7: open[7] = m.ev; break;
+      case 6: t.h2d_ms += elapsed(h, open[7], m.ev); break;
+      case 0: open[0] = m.ev; break;
+      case 1: t.fstage_ms += elapsed(h, open[0], m.ev); break;
+      case 2: open[2] = m.ev; break;
+      case 3: t.xstage_ms += elapsed(h, open[2], m.ev); break;
+      case 4: open[4] = m.ev; break;
+      case 5: t.finalise_ms += elapsed(h, open[4], m.ev); break;
+    }
   }
-  t.finalise_ms = elapsed(h, 1 + 3 * n, 2 + 3 * n);
-  t.total_ms = elapsed(h, 0, 2 + 3 * n);
+  t.total_ms = elapsed(h, first, last);
+}
+
+void destroy_graph(gxc_handle *h) {
+  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+  h->graph_exec = nullptr;
 }
 
 }  // namespace
@@ -369,6 +454,8 @@ void gxc_config_init(gxc_config *cfg) {
   cfg->npol = 2;
   cfg->is_complex = 0;
   cfg->precision = GXC_PREC_FP32;
+  cfg->overlap = 1;
+  cfg->graph = 1;
 }
 
 const char *gxc_status_string(gxc_status s) {
@@ -385,9 +472,7 @@ const char *gxc_status_string(gxc_status s) {
   return "unknown";
 }
 
-const char *gxc_last_error(const gxc_handle *h) {
-  return h ? h->err.c_str() : g_thread_error.c_str();
-}
+const char *gxc_last_error(const gxc_handle *h) { return h ? h->err.c_str() : g_thread_error.c_str(); }
 
 size_t gxc_station_bytes(const gxc_config *cfg) {
   Derived d;
@@ -410,6 +495,12 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   std::string why;
   gxc_status s = derive(cfg, &d, &why);
   if (s != GXC_OK) return set_err(nullptr, s, why);
+  const int out_mode = out_mode_of(cfg->precision);
+  const int ct = gxc::xm_channels_per_group(d.M, out_mode);
+  if (cfg->nchan % ct != 0)
+    return set_err(nullptr, GXC_ERR_UNSUPPORTED,
+                   "nchan must be a multiple of " + std::to_string(ct) + " for " + std::to_string(cfg->nant) +
+                       " stations at this precision");
   int dev = 0;
   s = check_device(nullptr, &dev);
   if (s != GXC_OK) return s;
@@ -417,20 +508,26 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   h->cfg = *cfg;
   h->d = d;
   h->device = dev;
+  h->out_mode = out_mode;
   auto fail = [&](gxc_status st) {
     g_thread_error = h->err;
     gxc_destroy(h);
     return st;
   };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+  // Default priority streams only.
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->xstream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_err(h, GXC_ERR_HIP, "hipStreamCreate failed"));
+  hipEvent_t *evs[6] = {&h->ev_fork, &h->ev_join, &h->ev_f[0], &h->ev_f[1], &h->ev_x[0], &h->ev_x[1]};
+  for (hipEvent_t *e : evs)
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(h, GXC_ERR_HIP, "hipEventCreate failed"));
   if ((s = upload_tables(h)) != GXC_OK) return fail(s);
   const int nant = cfg->nant, nchan = cfg->nchan;
   h->h_delays.assign((size_t)nant * 4, 0.0);
   if ((s = dmalloc(h, &h->d_delays, (size_t)nant * 4)) != GXC_OK) return fail(s);
   if ((s = dmalloc(h, &h->d_station_ptrs, (size_t)nant)) != GXC_OK) return fail(s);
   if ((s = dmalloc(h, &h->d_valid, (size_t)nant * cfg->numffts)) != GXC_OK) return fail(s);
-  // Rows and their station pairs.
   std::vector<int2> rm = make_rowmap(nant, cfg->autos != 0);
   if ((s = dmalloc(h, &h->d_rowmap, rm.size())) != GXC_OK) return fail(s);
   if (!rm.empty() &&
@@ -440,26 +537,32 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   if ((s = dmalloc(h, &h->d_vis, (size_t)d.nrows * 4 * nchan)) != GXC_OK) return fail(s);
   if (hipMemset(h->d_vis, 0, (size_t)d.nrows * 4 * nchan * sizeof(float2)) != hipSuccess)
     return fail(set_err(h, GXC_ERR_HIP, "hipMemset failed"));
-  // Chunk of FFTs per device pass: bounded spectra buffer (default 8 GiB).
-  const size_t per_fft = (size_t)d.M * nchan * sizeof(float2);
-  long long chunk = cfg->chunk_ffts > 0 ? cfg->chunk_ffts : (long long)((8ull << 30) / per_fft);
-  if (chunk < 1) chunk = 1;
-  if (chunk > cfg->numffts) chunk = cfg->numffts;
-  if (cfg->precision == GXC_PREC_FP32) {
-    if ((s = ensure_spectra(h, (int)chunk)) != GXC_OK) return fail(s);
-    h->sb = gxc::xstage_block_stations(nant);
-    h->nb = (nant + h->sb - 1) / h->sb;
-    h->ntiles = h->nb * (h->nb + 1) / 2;
-    h->splits = choose_splits(nchan, h->ntiles, (int)chunk);
-    const size_t ni = 2 * (size_t)h->sb;
-    if ((s = dmalloc(h, &h->d_partial, (size_t)h->splits * h->ntiles * ni * ni * nchan)) != GXC_OK)
+  // Spectra buffers: one chunk each, two when overlapped.  Automatic chunks:
+  // the whole subint in one pass without overlap (bounded by 8 GiB), eight
+  // passes with overlap (bounded by 4 GiB per buffer).
+  h->nbuf = cfg->overlap ? 2 : 1;
+  const size_t eb = gxc::xm_element_bytes(out_mode);
+  const size_t per_fft = (size_t)d.M * nchan * eb;
+  long long chunk = cfg->chunk_ffts;
+  if (chunk <= 0) {
+    const size_t budget = cfg->overlap ? (4ull << 30) : (8ull << 30);
+    chunk = cfg->overlap ? (cfg->numffts + 7) / 8 : cfg->numffts;
+    chunk = std::min<long long>(chunk, (long long)(budget / per_fft));
+  }
+  chunk = std::max<long long>(1, std::min<long long>(chunk, cfg->numffts));
+  if (chunk >= cfg->numffts) h->nbuf = 1;  // one pass: nothing to overlap
+  h->chunk = (int)chunk;
+  for (int b = 0; b < h->nbuf; b++)
+    if ((s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_spec[b]), (size_t)h->chunk * per_fft)) !=
+        GXC_OK)
       return fail(s);
-  } else {
-    h->chunk = (int)chunk;
-    std::string merr;
-    s = gxc::mixed_create(&h->mixed, *cfg, d.M, d.nrows, d.nbl, (int)chunk, h->d_valid, h->d_rowmap,
-                          h->d_counts, h->d_vis, &merr);
-    if (s != GXC_OK) return fail(set_err(h, s, merr));
+  h->splits = gxc::xm_choose_splits(d.M, nchan, h->chunk, out_mode);
+  if ((s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_partial),
+                   gxc::xm_partial_bytes(d.M, nchan, h->splits))) != GXC_OK)
+    return fail(s);
+  if (out_mode != gxc::OUT_F32) {
+    if ((s = dmalloc(h, &h->d_scales, (size_t)d.M * nchan)) != GXC_OK) return fail(s);
+    if ((s = dmalloc(h, &h->d_power, (size_t)gxc::kPowerRows * d.M * nchan)) != GXC_OK) return fail(s);
   }
   *out = h;
   return GXC_OK;
@@ -468,9 +571,16 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
 void gxc_destroy(gxc_handle *h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
-  for (hipEvent_t e : h->events) hipEventDestroy(e);
-  gxc::mixed_destroy(h->mixed);
-  if (h->cfg.precision == GXC_PREC_FP32 && h->d_spectra) hipFree(h->d_spectra);
+  if (h->xstream) hipStreamSynchronize(h->xstream);
+  destroy_graph(h);
+  for (hipEvent_t e : h->tev) hipEventDestroy(e);
+  hipEvent_t evs[6] = {h->ev_fork, h->ev_join, h->ev_f[0], h->ev_f[1], h->ev_x[0], h->ev_x[1]};
+  for (hipEvent_t e : evs)
+    if (e) hipEventDestroy(e);
+  for (void *p : h->d_spec) hipFree(p);
+  hipFree(h->d_partial);
+  hipFree(h->d_scales);
+  hipFree(h->d_power);
   hipFree(h->d_input_owned);
   hipFree(h->d_station_ptrs);
   hipFree(h->d_delays);
@@ -478,25 +588,28 @@ void gxc_destroy(gxc_handle *h) {
   hipFree(h->d_subfreq);
   hipFree(h->d_stepfreq);
   hipFree(h->d_valid);
-  hipFree(h->d_partial);
   hipFree(h->d_counts);
   hipFree(h->d_rowmap);
   hipFree(h->d_vis);
+  if (h->xstream) hipStreamDestroy(h->xstream);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
 
 gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station) {
   if (!h || !per_station) return set_err(h, GXC_ERR_INVALID_ARG, "null input");
-  h->host_in.assign(per_station, per_station + h->cfg.nant);
-  for (const uint8_t *p : h->host_in)
+  std::vector<const uint8_t *> in(per_station, per_station + h->cfg.nant);
+  for (const uint8_t *p : in)
     if (!p) return set_err(h, GXC_ERR_INVALID_ARG, "null station buffer");
-  if (h->input_device && h->d_input_owned) {
-    // switching back from device input: restore pointers into the owned copy
+  GXC_HIP(h, hipStreamSynchronize(h->stream));
+  h->host_in = in;
+  if (h->input_device || !h->d_input_owned) {
+    h->input_device = false;
+    GXC_TRY(ensure_owned_input(h));
+    // restore the station pointers to the owned copy
     std::vector<const uint8_t *> ptrs(h->cfg.nant);
     for (int a = 0; a < h->cfg.nant; a++) ptrs[a] = h->d_input_owned + (size_t)a * h->d.station_bytes;
-    GXC_HIP(h, hipMemcpy(h->d_station_ptrs, ptrs.data(), h->cfg.nant * sizeof(void *),
-                         hipMemcpyHostToDevice));
+    GXC_HIP(h, hipMemcpy(h->d_station_ptrs, ptrs.data(), h->cfg.nant * sizeof(void *), hipMemcpyHostToDevice));
   }
   h->input_device = false;
   h->input_set = true;
@@ -505,12 +618,12 @@ gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station) {
 
 gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station) {
   if (!h || !per_station) return set_err(h, GXC_ERR_INVALID_ARG, "null input");
-  h->dev_in.assign(per_station, per_station + h->cfg.nant);
-  for (const void *p : h->dev_in)
+  std::vector<const void *> in(per_station, per_station + h->cfg.nant);
+  for (const void *p : in)
     if (!p) return set_err(h, GXC_ERR_INVALID_ARG, "null station buffer");
   GXC_HIP(h, hipStreamSynchronize(h->stream));
-  GXC_HIP(h, hipMemcpy(h->d_station_ptrs, h->dev_in.data(), h->cfg.nant * sizeof(void *),
-                       hipMemcpyHostToDevice));
+  h->dev_in = in;
+  GXC_HIP(h, hipMemcpy(h->d_station_ptrs, h->dev_in.data(), h->cfg.nant * sizeof(void *), hipMemcpyHostToDevice));
   h->input_device = true;
   h->input_set = true;
   return GXC_OK;
@@ -518,8 +631,9 @@ gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station) {
 
 gxc_status gxc_set_delays(gxc_handle *h, const double *const *poly3, const double *fileoffsets) {
   if (!h || !poly3 || !fileoffsets) return set_err(h, GXC_ERR_INVALID_ARG, "null delays");
-  for (int a = 0; a < h->cfg.nant; a++) {
+  for (int a = 0; a < h->cfg.nant; a++)
     if (!poly3[a]) return set_err(h, GXC_ERR_INVALID_ARG, "null delay polynomial");
+  for (int a = 0; a < h->cfg.nant; a++) {
     for (int i = 0; i < 3; i++) h->h_delays[4 * a + i] = poly3[a][i];
     h->h_delays[4 * a + 3] = fileoffsets[a];
   }
@@ -618,25 +732,17 @@ gxc_status gxc_fstage(gxc_handle *h, int t0, int nfft, gxc_cf32 *spectra, uint8_
   if (t0 < 0 || nfft < 1 || t0 + nfft > h->cfg.numffts)
     return set_err(h, GXC_ERR_INVALID_ARG, "FFT range outside the subintegration");
   if (!h->input_set || !h->delays_set) return set_err(h, GXC_ERR_STATE, "input or delays not set");
-  if (!h->input_device) {
-    if (!h->d_input_owned) {
-      GXC_TRY(dmalloc(h, &h->d_input_owned, h->d.station_bytes * (size_t)h->cfg.nant));
-      std::vector<const uint8_t *> ptrs(h->cfg.nant);
-      for (int a = 0; a < h->cfg.nant; a++)
-        ptrs[a] = h->d_input_owned + (size_t)a * h->d.station_bytes;
-      GXC_HIP(h, hipMemcpy(h->d_station_ptrs, ptrs.data(), h->cfg.nant * sizeof(void *),
-                           hipMemcpyHostToDevice));
-    }
-    for (int a = 0; a < h->cfg.nant; a++)
-      GXC_HIP(h, hipMemcpyAsync(h->d_input_owned + (size_t)a * h->d.station_bytes, h->host_in[a],
-                                h->d.station_bytes, hipMemcpyHostToDevice, h->stream));
-  }
-  GXC_TRY(ensure_spectra(h, nfft));
-  GXC_TRY(run_fstage(h, t0, nfft));
-  GXC_HIP(h, hipStreamSynchronize(h->stream));
+  GXC_TRY(enqueue_input_copy(h));
   const size_t per_fft = (size_t)h->d.M * h->cfg.nchan;
-  if (spectra)
-    GXC_HIP(h, hipMemcpy(spectra, h->d_spectra, per_fft * nfft * sizeof(float2), hipMemcpyDeviceToHost));
+  float2 *tmp = nullptr;
+  GXC_TRY(dmalloc(h, &tmp, per_fft * nfft));
+  gxc_status s = run_fstage(h, t0, nfft, gxc::OUT_F32, tmp, 1, h->stream);
+  if (s == GXC_OK && hipStreamSynchronize(h->stream) != hipSuccess) s = set_err(h, GXC_ERR_HIP, "F-stage failed");
+  if (s == GXC_OK && spectra &&
+      hipMemcpy(spectra, tmp, per_fft * nfft * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess)
+    s = set_err(h, GXC_ERR_HIP, "copy of spectra failed");
+  hipFree(tmp);
+  GXC_TRY(s);
   if (valid)
     for (int a = 0; a < h->cfg.nant; a++)
       GXC_HIP(h, hipMemcpy(valid + (size_t)a * nfft, h->d_valid + (size_t)a * h->cfg.numffts + t0, nfft,
@@ -648,24 +754,31 @@ gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft, i
                        gxc_cf32 *out) {
   if (!spectra || !out || nant < 1 || nchan < 1 || nfft < 1)
     return set_err(nullptr, GXC_ERR_INVALID_ARG, "bad xengine arguments");
+  if (precision < GXC_PREC_FP32 || precision > GXC_PREC_FP8)
+    return set_err(nullptr, GXC_ERR_INVALID_ARG, "unknown precision");
+  const int M = 2 * nant, mode = out_mode_of(precision);
+  if (gxc::xm_blocks(M) == 0 || nchan % gxc::xm_channels_per_group(M, mode) != 0)
+    return set_err(nullptr, GXC_ERR_UNSUPPORTED, "X-engine shape not supported");
   int dev = 0;
   GXC_TRY(check_device(nullptr, &dev));
-  if (precision != GXC_PREC_FP32)
-    return gxc::mixed_xengine(spectra, nant, nchan, nfft, autos, precision, out, &g_thread_error);
   gxc_handle tmp;
   gxc_handle *h = &tmp;
   h->cfg.nant = nant;
   h->cfg.nchan = nchan;
   h->cfg.numffts = nfft;
   h->cfg.autos = autos;
-  h->d.M = 2 * nant;
+  h->out_mode = mode;
+  h->d.M = M;
   h->d.nbl = nant * (nant - 1) / 2;
   h->d.nrows = h->d.nbl + (autos ? nant : 0);
+  float2 *d_in = nullptr;
   struct Cleanup {
     gxc_handle *h;
+    float2 **in;
     ~Cleanup() {
       if (h->stream) hipStreamSynchronize(h->stream);
-      hipFree(h->d_spectra);
+      hipFree(*in);
+      hipFree(h->d_spec[0]);
       hipFree(h->d_partial);
       hipFree(h->d_valid);
       hipFree(h->d_counts);
@@ -673,11 +786,13 @@ gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft, i
       hipFree(h->d_vis);
       if (h->stream) hipStreamDestroy(h->stream);
     }
-  } cleanup{h};
+  } cleanup{h, &d_in};
   GXC_HIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  GXC_TRY(ensure_spectra(h, nfft));
-  const size_t nspec = (size_t)nfft * h->d.M * nchan;
-  GXC_HIP(h, hipMemcpy(h->d_spectra, spectra, nspec * sizeof(float2), hipMemcpyHostToDevice));
+  const size_t nspec = (size_t)nfft * M * nchan;
+  GXC_TRY(dmalloc(h, &d_in, nspec));
+  GXC_HIP(h, hipMemcpy(d_in, spectra, nspec * sizeof(float2), hipMemcpyHostToDevice));
+  GXC_TRY(dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_spec[0]), nspec * gxc::xm_element_bytes(mode)));
+  GXC_HIP(h, gxc::launch_quantise(h->d_spec[0], d_in, nspec, mode, h->stream));
   GXC_TRY(dmalloc(h, &h->d_valid, (size_t)nant * nfft));
   GXC_HIP(h, hipMemset(h->d_valid, 1, (size_t)nant * nfft));
   std::vector<int2> rm = make_rowmap(nant, autos != 0);
@@ -685,14 +800,12 @@ gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft, i
   if (!rm.empty()) GXC_HIP(h, hipMemcpy(h->d_rowmap, rm.data(), rm.size() * sizeof(int2), hipMemcpyHostToDevice));
   GXC_TRY(dmalloc(h, &h->d_counts, (size_t)h->d.nrows));
   GXC_TRY(dmalloc(h, &h->d_vis, (size_t)h->d.nrows * 4 * nchan));
-  h->sb = gxc::xstage_block_stations(nant);
-  h->nb = (nant + h->sb - 1) / h->sb;
-  h->ntiles = h->nb * (h->nb + 1) / 2;
-  h->splits = choose_splits(nchan, h->ntiles, nfft);
-  const size_t ni = 2 * (size_t)h->sb;
-  GXC_TRY(dmalloc(h, &h->d_partial, (size_t)h->splits * h->ntiles * ni * ni * nchan));
-  GXC_TRY(run_xstage(h, nfft, 0));
-  GXC_TRY(run_finalise(h));
+  h->splits = gxc::xm_choose_splits(M, nchan, nfft, mode);
+  GXC_TRY(dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_partial), gxc::xm_partial_bytes(M, nchan, h->splits)));
+  GXC_TRY(run_xstage(h, h->d_spec[0], nfft, 0, h->stream));
+  // unit scales: the spectra were packed without scaling
+  h->out_mode = gxc::OUT_F32;
+  GXC_TRY(run_finalise(h, h->stream));
   GXC_HIP(h, hipStreamSynchronize(h->stream));
   GXC_HIP(h, hipMemcpy(out, h->d_vis, (size_t)h->d.nrows * 4 * nchan * sizeof(float2), hipMemcpyDeviceToHost));
   return GXC_OK;

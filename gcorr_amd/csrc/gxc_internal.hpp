@@ -30,43 +30,34 @@ struct Geometry {
   }
 };
 
+// What the F-stage writes per (station, FFT): FP32 spectra, scaled
+// reduced-precision spectra (complex: 2 x 16 bits or 2 x 8 bits), or |X|^2
+// accumulated per (input, channel) for the scale estimate.
+enum OutMode { OUT_F32 = 0, OUT_F16 = 1, OUT_BF16 = 2, OUT_FP8 = 3, OUT_POWER = 4 };
+
 // Arguments of the F-stage kernel (unpack, fringe rotate, FFT, fractional
-// sample correction) for FFTs [t0, t0+nt) of every station.
+// sample correction) for FFTs t0 + i*tstride, i in [0, nt), of every station.
 struct FParams {
   const uint8_t *const *station;  // [nant] device pointers to packed input
   const double *delays;           // [nant][4]: d0, d1, d2, file offset (s)
   const float *subchanfreq;       // [s]          fxkernel.cpp:173
   const float *stepchanfreq;      // [fracsteps]  fxkernel.cpp:174
   const float2 *twiddle;          // [L] exp(-2 pi i m / L)
-  float2 *spectra;                // [nt][nant*2][nchan]
+  float2 *spectra;                // OUT_F32: [nt][nant*2][nchan]
+  void *out;                      // OUT_F16/BF16/FP8: [nt][nant*2][nchan] packed complex
+  const float *scales;            // [nant*2][nchan] multipliers before rounding
+  float *power;                   // OUT_POWER: [nt][nant*2][nchan] |X|^2 per sampled FFT
   uint8_t *valid;                 // [nant][numffts]
-  int t0, nt, nant, numffts;
+  int out_mode;
+  int t0, nt, tstride, nant, numffts;
   int maxoffset;                  // (numffts-1)*L
   int stepmul;                    // s (stride law) or S2 (exact phase)
   int fraclo;                     // LO has a fractional part
   double lo, lofrac, sampletime;
 };
 
-// Arguments of the FP32 X-stage kernel.
-struct XParams {
-  const float2 *spectra;  // [nt][2*nant][nchan]
-  float2 *partial;        // [splits][ntiles][(2*SB)^2][nchan]
-  int nt, nant, nchan;
-  int nb, ntiles;         // station blocks and block-pair tiles
-  int tile0;              // first tile of this launch's tile group
-  int splits;
-  int accumulate;         // 1: add into partial, 0: overwrite
-};
-
-// Rows of the output: station pair (j,k) per row; autos have j == k.
-struct FinParams {
-  const float2 *partial;  // X-stage partials
-  const int2 *rowmap;     // [nrows]
-  const int *counts;      // [nrows]
-  float2 *vis;            // [nrows][4][nchan]
-  int nrows, nchan, nb, ntiles, splits, sb;
-};
-
+// Row counts: for each output row (station pair; autos have j == k) the
+// number of FFTs in which both stations were valid.
 struct CountParams {
   const uint8_t *valid;  // [nant][numffts]
   const int2 *rowmap;    // [nrows]
@@ -77,9 +68,6 @@ struct CountParams {
 // Launchers (defined in the .hip files).  Return hipSuccess or the launch error.
 bool fstage_supported(int L, bool cplx);
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s);
-int xstage_block_stations(int nant);
-hipError_t launch_xstage(const XParams &p, int sb, hipStream_t s);
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
-hipError_t launch_finalise(const FinParams &p, hipStream_t s);
 
 }  // namespace gxc

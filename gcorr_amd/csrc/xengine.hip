@@ -1,0 +1,399 @@
+// MFMA X-engine for gfx950: complex Hermitian rank-k update of each
+// channel's inputs with FP32 accumulation, for FP32, FP16, BF16 and FP8
+// (e4m3) spectra.
+//
+// For channel c over FFTs t, with x_a = a_a + i b_a (a = input index):
+//   V_ab = sum_t x_a conj(x_b)   (fxkernel.cpp:349, X_j,l * conj(X_k,m))
+//   Re V = R R^T + I I^T,  Im V_ab = G_IR[a][b] - G_IR[b][a],  G_IR = I R^T
+// where R, I are the [M x T] real and imaginary parts.  Each product is a
+// 16x16 MFMA over 16-input blocks: v_mfma_f32_16x16x4f32 (K = 4 FFTs) for
+// FP32, v_mfma_f32_16x16x32_{f16,bf16,fp8_fp8} (K = 32) otherwise.
+//
+// The F-stage writes spectra channel-contiguous ([t][input][chan]), the
+// layout its last FFT pass stores coalesced.  Each workgroup stages a
+// [TB FFTs][M][CT channels] tile through LDS, transposed per channel so
+// that a lane's MFMA operand is contiguous there: [plane][input][t] for the
+// 8-element operands of the 16/8-bit types, [plane][t][input] for FP32.
+// Rows of inputs past M stay zero in LDS.
+#include <type_traits>
+
+#include "gxc_xengine.hpp"
+
+namespace gxc {
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int ET, int MB>
+struct XM {
+  static constexpr bool F32 = ET == OUT_F32;
+  static constexpr int EB = F32 ? 4 : ET == OUT_FP8 ? 1 : 2;  // bytes per real component
+  static constexpr int CB = 2 * EB;                          // bytes per complex value
+  static constexpr int K = F32 ? 4 : 32;                     // FFTs per MFMA
+  static constexpr int TB = F32 ? 16 : 32;                   // FFTs per LDS stage
+  static constexpr int MP = 16 * MB;                         // inputs padded to blocks
+  static constexpr int CT = (F32 ? 16 : 32) / MB;            // channels per workgroup
+  static constexpr int PLANEB = MP * TB * EB;
+  static constexpr int CHB = 2 * PLANEB + 16;  // +16 B spreads channels over banks
+  static constexpr int LDSB = CT * CHB;
+  static constexpr int NTILE = MB * (MB + 1) / 2;
+  static constexpr int EPL = 16 / CB;   // complex values per 16-byte load
+  static constexpr int TPR = CT / EPL;  // 16-byte loads per (t, input) row
+  static constexpr int CPW = CT / 4;    // channels per wave
+  // byte offset of component (plane, input m, FFT t) within a channel
+  __device__ static constexpr int at(int plane, int m, int t) {
+    return plane * PLANEB + (F32 ? (t * MP + m) : (m * TB + t)) * EB;
+  }
+};
+
+__device__ __forceinline__ floatx4 mfma(const uint4 &a, const uint4 &b, floatx4 c,
+                                        std::integral_constant<int, OUT_F16>) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b),
+                                                 c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma(const uint4 &a, const uint4 &b, floatx4 c,
+                                        std::integral_constant<int, OUT_BF16>) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma(const uint2 &a, const uint2 &b, floatx4 c,
+                                        std::integral_constant<int, OUT_FP8>) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(__builtin_bit_cast(long, a), __builtin_bit_cast(long, b),
+                                                     c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma(const float &a, const float &b, floatx4 c,
+                                        std::integral_constant<int, OUT_F32>) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int ET>
+struct Operand {
+  using type = typename std::conditional<
+      ET == OUT_F32, float, typename std::conditional<ET == OUT_FP8, uint2, uint4>::type>::type;
+};
+
+// Scatter one 16-byte load (EPL complex values of consecutive channels at
+// FFT t, input m) into the per-channel transposed LDS layout.
+template <int ET, int MB>
+__device__ __forceinline__ void stage_store(unsigned char *lds, const uint4 &v, int ch0, int m, int t) {
+  using X = XM<ET, MB>;
+  const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+  unsigned char *dst = lds + ch0 * X::CHB;
+#pragma unroll
+  for (int e = 0; e < X::EPL; e++) {
+    unsigned char *d = dst + e * X::CHB;
+    if constexpr (X::EB == 4) {
+      *reinterpret_cast<uint32_t *>(d + X::at(0, m, t)) = w4[2 * e];
+      *reinterpret_cast<uint32_t *>(d + X::at(1, m, t)) = w4[2 * e + 1];
+    } else if constexpr (X::EB == 2) {
+      *reinterpret_cast<uint16_t *>(d + X::at(0, m, t)) = (uint16_t)(w4[e] & 0xffff);
+      *reinterpret_cast<uint16_t *>(d + X::at(1, m, t)) = (uint16_t)(w4[e] >> 16);
+    } else {
+      const uint32_t word = w4[e >> 1] >> (16 * (e & 1));
+      d[X::at(0, m, t)] = (unsigned char)(word & 0xff);
+      d[X::at(1, m, t)] = (unsigned char)((word >> 8) & 0xff);
+    }
+  }
+}
+
+template <int ET, int MB>
+__global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
+  using X = XM<ET, MB>;
+  using Op = typename Operand<ET>::type;
+  constexpr int NBATCH = 8;  // 16-byte loads issued before their LDS stores
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = blockIdx.x * X::CT;
+  const int split = blockIdx.y;
+  const int tb = (int)((long long)p.nt * split / p.splits);
+  const int te = (int)((long long)p.nt * (split + 1) / p.splits);
+  const int M = p.M;
+
+  for (int i = tid * 16; i < X::LDSB; i += 256 * 16) *reinterpret_cast<uint4 *>(lds + i) = make_uint4(0, 0, 0, 0);
+
+  floatx4 acc_re[X::CPW][X::NTILE], acc_ir[X::CPW][X::NTILE], acc_ri[X::CPW][X::NTILE];
+#pragma unroll
+  for (int w = 0; w < X::CPW; w++)
+#pragma unroll
+    for (int t = 0; t < X::NTILE; t++) acc_re[w][t] = acc_ir[w][t] = acc_ri[w][t] = floatx4{0, 0, 0, 0};
+
+  const unsigned char *src = static_cast<const unsigned char *>(p.spectra);
+  const size_t rowbytes = (size_t)p.nchan * X::CB;
+  const int nload = X::TB * M * X::TPR;  // 16-byte loads per stage
+  for (int ts = tb; ts < te; ts += X::TB) {
+    __syncthreads();  // the previous stage's operand reads are done
+    for (int i0 = tid; i0 < nload; i0 += 256 * NBATCH) {
+      uint4 v[NBATCH];
+#pragma unroll
+      for (int u = 0; u < NBATCH; u++) {
+        const int idx = i0 + u * 256;
+        v[u] = make_uint4(0, 0, 0, 0);
+        if (idx < nload) {
+          const int piece = idx % X::TPR, row = idx / X::TPR;
+          const int t = row / M, m = row - t * M;
+          if (ts + t < te)
+            v[u] = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
+                                                    (size_t)(c0 + piece * X::EPL) * X::CB);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NBATCH; u++) {
+        const int idx = i0 + u * 256;
+        if (idx < nload) {
+          const int piece = idx % X::TPR, row = idx / X::TPR;
+          const int t = row / M, m = row - t * M;
+          stage_store<ET, MB>(lds, v[u], piece * X::EPL, m, t);
+        }
+      }
+    }
+    __syncthreads();
+    // Wave w: channels w, w+4, ...  Lane l's operand: input l%16 of a block
+    // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
+#pragma unroll
+    for (int w = 0; w < X::CPW; w++) {
+      const unsigned char *ch = lds + (wave + 4 * w) * X::CHB;
+#pragma unroll
+      for (int ks = 0; ks < X::TB / X::K; ks++) {
+        Op R[MB], I[MB];
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+          const int m = b * 16 + (lane & 15);
+          const int t = X::F32 ? ks * X::K + (lane >> 4) : (lane >> 4) * 8;
+          R[b] = *reinterpret_cast<const Op *>(ch + X::at(0, m, t));
+          I[b] = *reinterpret_cast<const Op *>(ch + X::at(1, m, t));
+        }
+        int tile = 0;
+#pragma unroll
+        for (int bi = 0; bi < MB; bi++)
+#pragma unroll
+          for (int bj = bi; bj < MB; bj++, tile++) {
+            const std::integral_constant<int, ET> et;
+            acc_re[w][tile] = mfma(R[bi], R[bj], acc_re[w][tile], et);
+            acc_re[w][tile] = mfma(I[bi], I[bj], acc_re[w][tile], et);
+            acc_ir[w][tile] = mfma(I[bi], R[bj], acc_ir[w][tile], et);
+            if (bi != bj) acc_ri[w][tile] = mfma(R[bi], I[bj], acc_ri[w][tile], et);
+          }
+      }
+    }
+  }
+
+  // D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4.
+  const size_t MM = (size_t)M * M;
+#pragma unroll
+  for (int w = 0; w < X::CPW; w++) {
+    const int c = c0 + wave + 4 * w;
+    float *pre = p.partial + ((size_t)split * p.nchan + c) * 2 * MM;
+    float *pir = pre + MM;
+    int tile = 0;
+#pragma unroll
+    for (int bi = 0; bi < MB; bi++)
+#pragma unroll
+      for (int bj = bi; bj < MB; bj++, tile++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int a = bi * 16 + 4 * (lane >> 4) + r;
+          const int b = bj * 16 + (lane & 15);
+          if (a < M && b < M) {
+            float vre = acc_re[w][tile][r], vir = acc_ir[w][tile][r];
+            if (p.accumulate) {
+              vre += pre[(size_t)a * M + b];
+              vir += pir[(size_t)a * M + b];
+            }
+            pre[(size_t)a * M + b] = vre;
+            pir[(size_t)a * M + b] = vir;
+            if (bi != bj) {  // R_bi I_bj^T holds G_IR transposed
+              float vri = acc_ri[w][tile][r];
+              if (p.accumulate) vri += pir[(size_t)b * M + a];
+              pir[(size_t)b * M + a] = vri;
+            }
+          }
+        }
+      }
+  }
+}
+
+__device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
+#pragma clang fp contract(off)
+  const float t = sum + v;
+  if (fabsf(sum) >= fabsf(v))
+    comp += (sum - t) + v;
+  else
+    comp += (v - t) + sum;
+  sum = t;
+}
+
+__global__ void __launch_bounds__(256) finalise_mixed_kernel(FinMParams p) {
+  // Compensated sum of the split partials, Im from the antisymmetric part of
+  // G_IR, undo the per-(input, channel) scales, normalise by the row count
+  // (fxkernel.cpp:358-367).
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int prod = blockIdx.y, row = blockIdx.z;
+  if (c >= p.nchan) return;
+  const int2 jk = p.rowmap[row];
+  const int a = 2 * jk.x + (prod >> 1), b = 2 * jk.y + (prod & 1);
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const size_t MM = (size_t)p.M * p.M;
+  float sre = 0.f, cre = 0.f, sim = 0.f, cim = 0.f;
+  for (int s = 0; s < p.splits; s++) {
+    const float *pre = p.partial + ((size_t)s * p.nchan + c) * 2 * MM;
+    const float *pir = pre + MM;
+    neumaier(sre, cre, pre[(size_t)lo * p.M + hi]);
+    neumaier(sim, cim, pir[(size_t)a * p.M + b] - pir[(size_t)b * p.M + a]);
+  }
+  float re = sre + cre, im = sim + cim;
+  if (p.scales) {
+    const float sc = p.scales[(size_t)a * p.nchan + c] * p.scales[(size_t)b * p.nchan + c];
+    re /= sc;
+    im /= sc;
+  }
+  const int cnt = p.counts[row];
+  if (cnt != 0) {
+    re /= (float)cnt;
+    im /= (float)cnt;
+  }
+  p.vis[((size_t)row * 4 + prod) * p.nchan + c] = make_float2(re, im);
+}
+
+__global__ void fill_kernel(float *p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void scales_kernel(float *scales, const float *power, int n, int nrows, float target) {
+  // Mean |X|^2 over the sampled FFTs (rows summed in order), then the power
+  // of two nearest target/rms: scaling and unscaling are exact, and the
+  // scale is insensitive to the last bits of the estimate.
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float sum = 0.f;
+  for (int r = 0; r < nrows; r++) sum += power[(size_t)r * n + i];
+  const float pw = sum / (float)nrows;
+  scales[i] = pw > 0.f ? exp2f(rintf(log2f(target / sqrtf(pw)))) : 1.0f;
+}
+
+__device__ __forceinline__ uint32_t q_bf16_1(float x) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+__global__ void quantise_kernel(void *out, const float2 *in, size_t n, int mode) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = in[i];
+  if (mode == OUT_F16) {
+    const _Float16 a = (_Float16)v.x, b = (_Float16)v.y;
+    static_cast<uint32_t *>(out)[i] =
+        (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+  } else if (mode == OUT_BF16) {
+    static_cast<uint32_t *>(out)[i] = q_bf16_1(v.x) | (q_bf16_1(v.y) << 16);
+  } else if (mode == OUT_FP8) {
+    const float re = fminf(fmaxf(v.x, -448.f), 448.f), im = fminf(fmaxf(v.y, -448.f), 448.f);
+    static_cast<uint16_t *>(out)[i] = (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false) & 0xffff);
+  } else {
+    static_cast<float2 *>(out)[i] = v;
+  }
+}
+
+template <int ET, int MB>
+hipError_t launch_xm(const XMParams &p, hipStream_t s) {
+  using X = XM<ET, MB>;
+  if (p.nchan % X::CT) return hipErrorInvalidValue;
+  dim3 grid(p.nchan / X::CT, p.splits);
+  static bool attr_set = false;  // dynamic LDS above 64 KiB must be allowed once per kernel
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((xmfma_kernel<ET, MB>), grid, dim3(256), X::LDSB, s, p);
+  return hipGetLastError();
+}
+
+template <int ET>
+hipError_t launch_xm_mb(const XMParams &p, hipStream_t s) {
+  switch (xm_blocks(p.M)) {
+    case 1: return launch_xm<ET, 1>(p, s);
+    case 2: return launch_xm<ET, 2>(p, s);
+    case 4: return launch_xm<ET, 4>(p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+int xm_blocks(int M) {
+  if (M <= 16) return 1;
+  if (M <= 32) return 2;
+  if (M <= 64) return 4;
+  return 0;
+}
+
+int xm_channels_per_group(int M, int out_mode) {
+  const int mb = xm_blocks(M);
+  if (mb == 0) return 0;
+  return (out_mode == OUT_F32 ? 16 : 32) / mb;
+}
+
+int xm_choose_splits(int M, int nchan, int nt, int out_mode) {
+  const int ct = xm_channels_per_group(M, out_mode);
+  if (ct == 0) return 1;
+  const long long groups = nchan / ct;
+  const int tb = out_mode == OUT_F32 ? 16 : 32;
+  long long s = (1024 + groups - 1) / groups;  // ~4 workgroups per CU
+  const long long maxs = nt / (4 * tb) > 0 ? nt / (4 * tb) : 1;
+  if (s > maxs) s = maxs;
+  if (s > 256) s = 256;
+  return (int)(s < 1 ? 1 : s);
+}
+
+size_t xm_partial_bytes(int M, int nchan, int splits) {
+  return (size_t)splits * nchan * 2 * (size_t)M * M * sizeof(float);
+}
+
+int xm_element_bytes(int out_mode) { return out_mode == OUT_FP8 ? 2 : out_mode == OUT_F32 ? 8 : 4; }
+
+float xm_target_rms(int out_mode) {
+  // FP16/BF16 keep relative precision at any normal magnitude: unit RMS.
+  // e4m3 spans 2^-6..448: RMS 32 leaves ~14x headroom above 1 sigma and
+  // keeps 0.0005 sigma above the subnormal range.
+  return out_mode == OUT_FP8 ? 32.0f : 1.0f;
+}
+
+hipError_t launch_xmfma(const XMParams &p, int out_mode, hipStream_t s) {
+  switch (out_mode) {
+    case OUT_F32: return launch_xm_mb<OUT_F32>(p, s);
+    case OUT_F16: return launch_xm_mb<OUT_F16>(p, s);
+    case OUT_BF16: return launch_xm_mb<OUT_BF16>(p, s);
+    case OUT_FP8: return launch_xm_mb<OUT_FP8>(p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s) {
+  hipLaunchKernelGGL(scales_kernel, dim3((n + 255) / 256), dim3(256), 0, s, scales, power, n, nrows, target);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalise_mixed(const FinMParams &p, hipStream_t s) {
+  if (p.nrows <= 0) return hipSuccess;
+  dim3 grid((p.nchan + 255) / 256, 4, p.nrows);
+  hipLaunchKernelGGL(finalise_mixed_kernel, grid, dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantise(void *out, const float2 *in, size_t n, int out_mode, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(quantise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, in, n, out_mode);
+  return hipGetLastError();
+}
+
+}  // namespace gxc

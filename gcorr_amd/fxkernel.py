@@ -16,7 +16,7 @@ from ._lib import check
 
 
 def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact_phase,
-            chunk_ffts):
+            chunk_ffts, overlap=True, graph=True):
     c = _lib.Config()
     _lib.load().gxc_config_init(ctypes.byref(c))
     c.nant, c.nchan, c.numffts, c.nbit = int(nant), int(nchan), int(nfft), int(nbit)
@@ -27,6 +27,8 @@ def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact
     c.autos = int(bool(autos))
     c.exact_phase = int(bool(exact_phase))
     c.chunk_ffts = int(chunk_ffts)
+    c.overlap = int(bool(overlap))
+    c.graph = int(bool(graph))
     return c
 
 
@@ -44,10 +46,10 @@ class FxKernel:
     """FxKernel(nant, nchan, nfft, nbit, lo, bw, is_complex) on the MI355X path."""
 
     def __init__(self, nant, nchan, nfft, nbit, lo, bw, is_complex=False, *, precision="fp32",
-                 autos=False, exact_phase=False, chunk_ffts=0):
+                 autos=False, exact_phase=False, chunk_ffts=0, overlap=True, graph=True):
         self._L = _lib.load()
         self.cfg = _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos,
-                           exact_phase, chunk_ffts)
+                           exact_phase, chunk_ffts, overlap, graph)
         self.nant, self.nchan, self.numffts = int(nant), int(nchan), int(nfft)
         self.nrows = int(self._L.gxc_num_rows(ctypes.byref(self.cfg)))
         self.station_bytes = int(self._L.gxc_station_bytes(ctypes.byref(self.cfg)))

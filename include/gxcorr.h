@@ -80,7 +80,12 @@ typedef struct {
   int exact_phase; /* 0: fxkernel's stride fringe-phase law (default);
                       1: exact per-sample phase */
   int chunk_ffts;  /* FFTs per device pass; 0 = choose automatically */
-  int reserved[6];
+  int overlap;     /* 1: the F-stage of chunk k+1 runs on a second stream
+                      while the X-stage of chunk k runs (double-buffered
+                      spectra); 0: one stream, one chunk after another */
+  int graph;       /* 1: capture the device work of one subintegration as a
+                      hipGraph on first use and replay it afterwards */
+  int reserved[4];
 } gxc_config;
 
 typedef struct gxc_handle gxc_handle;
@@ -90,14 +95,17 @@ typedef struct gxc_handle gxc_handle;
 typedef struct {
   double h2d_ms;       /* host-to-device input copy (0 for device input) */
   double fstage_ms;    /* unpack+rotate+FFT+frac (all chunks) */
-  double xstage_ms;    /* cross-multiply-accumulate (all chunks) */
+  double xstage_ms;    /* cross-multiply-accumulate (all chunks); with overlap
+                          on, F and X times are measured on their own streams
+                          and overlap in wall time */
   double finalise_ms;  /* count + normalise */
   double total_ms;     /* first to last event */
   int fstage_launches; /* number of F-stage kernel launches */
   int xstage_launches;
 } gxc_timings;
 
-/* Fill cfg with defaults: nbit 2, npol 2, real, FP32, no autos, stride law. */
+/* Fill cfg with defaults: nbit 2, npol 2, real, FP32, no autos, stride law,
+ * automatic chunking, overlap and graph replay on. */
 void gxc_config_init(gxc_config *cfg);
 
 /* FxKernel::FxKernel (fxkernel/src/fxkernel.cpp:30-180) */

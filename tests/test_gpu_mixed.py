@@ -1,0 +1,119 @@
+"""Mixed-precision path: FP16 / BF16 / FP8 e4m3 spectra with the MFMA X-engine.
+
+Exactness where the arithmetic allows it (integer spectra in [-5, 5] are
+representable in all three formats and their products sum exactly in FP32),
+and errors against the FP64 oracle on the generated signal otherwise.
+"""
+import numpy as np
+import pytest
+
+import gcorr_amd
+from gcorr_amd import metrics, synth
+from oracle import fxref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+@pytest.mark.parametrize("nant,nchan,nfft", [(3, 64, 100), (8, 32, 300), (12, 32, 37),
+                                             (16, 16, 64), (32, 8, 40)])
+def test_mfma_xengine_integer_kat(gpu, precision, nant, nchan, nfft):
+    """gcorr/test/unit.cc:73-159 pattern through the MFMA X-engine: 1, 2 and 4
+    input blocks of 16, a tail of FFTs not a multiple of 32."""
+    rng = np.random.default_rng(nant * 1000 + nfft)
+    X = (rng.integers(-5, 6, (nfft, 2 * nant, nchan))
+         + 1j * rng.integers(-5, 6, (nfft, 2 * nant, nchan))).astype(np.complex64)
+    got = gcorr_amd.xengine(X, nant, autos=True, precision=precision)
+    Xd = X.astype(np.complex128)
+    rows = [(j, k) for j in range(nant) for k in range(j + 1, nant)] + [(a, a) for a in range(nant)]
+    for r, (j, k) in enumerate(rows):
+        for l in range(2):
+            for m in range(2):
+                s = np.sum(Xd[:, 2 * j + l] * np.conj(Xd[:, 2 * k + m]), axis=0)
+                expect = (s.real.astype(np.float32) / np.float32(nfft)) \
+                    + 1j * (s.imag.astype(np.float32) / np.float32(nfft))
+                np.testing.assert_array_equal(got[r, 2 * l + m], expect.astype(np.complex64),
+                                              err_msg="row %d prod %d" % (r, 2 * l + m))
+
+
+_CACHE = {}
+
+
+def _case(w):
+    """Generated input and the FP64 oracle's visibilities, computed once per workload."""
+    if w not in _CACHE:
+        bufs, poly, offs = synth.generate(w, device="cpu")
+        data = [b.numpy() for b in bufs]
+        cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=w.nbit,
+                                is_complex=w.is_complex, autos=True)
+        ref, rc = fxref.process(cfg, data, poly, offs, "f64")
+        _CACHE[w] = (data, poly, offs, ref, rc)
+    return _CACHE[w]
+
+
+def _mixed_vs_oracle(w, precision):
+    data, poly, offs, ref, rc = _case(w)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                           precision=precision, autos=True)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    v, c = k.visibilities(), k.row_counts()
+    k.close()
+    np.testing.assert_array_equal(c, rc)
+    return v, ref
+
+
+# bound on eps vs the FP64 oracle: FP16 is the 1e-3 target; BF16 has 3 fewer
+# mantissa bits, e4m3 7 fewer.
+BOUND = {"fp16": 1e-3, "bf16": 8e-3, "fp8": 0.12}
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+def test_mixed_process_vs_fp64_oracle(gpu, precision):
+    """16 stations x 2 pols, 4096 channels with tones (the mixed-precision
+    workload's shape), 600 of its 31 250 FFTs."""
+    w = synth.WORKLOADS["c3"].scaled(600)
+    v, ref = _mixed_vs_oracle(w, precision)
+    e = metrics.eps(v, ref)
+    errs = metrics.amp_phase_errors(v, ref)
+    print(precision, "eps", e, errs)
+    assert e < BOUND[precision]
+
+
+def test_mixed_fp16_small_shapes(gpu):
+    for key, n in (("c1", 2000), ("c2", 300)):
+        w = synth.WORKLOADS[key].scaled(n)
+        v, ref = _mixed_vs_oracle(w, "fp16")
+        assert metrics.eps(v, ref) < 1e-3, key
+
+
+def test_mixed_chunked_matches_single_pass(gpu):
+    w = synth.WORKLOADS["c2"].scaled(400)
+    bufs, poly, offs = synth.generate(w, device="cpu")
+    data = [b.numpy() for b in bufs]
+    out = []
+    for chunk in (0, 97):
+        k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, precision="fp16",
+                               autos=True, chunk_ffts=chunk)
+        k.setInputData(data)
+        k.setDelays(poly, offs)
+        k.process()
+        out.append(k.visibilities())
+        k.close()
+    assert metrics.eps(out[1], out[0]) < 1e-5
+
+
+def test_mixed_is_deterministic(gpu):
+    w = synth.WORKLOADS["c1"].scaled(300)
+    data, poly, offs, _, _ = _case(w)
+    out = []
+    for _ in range(2):
+        k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, precision="fp16",
+                               autos=True)
+        k.setInputData(data)
+        k.setDelays(poly, offs)
+        k.process()
+        out.append(k.visibilities())
+        k.close()
+    np.testing.assert_array_equal(out[0], out[1])
