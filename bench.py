@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp8"])
     ap.add_argument("--numffts", type=int, default=0, help="override FFTs per subint (testing)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,

@@ -14,111 +14,19 @@
 //      loaded once for two FFTs.
 //   5. Last pass writes only bins < nchan, multiplied by the fractional
 //      sample rotator, as coalesced stores to spectra[t][2*station+pol][chan].
-#include "gxc_internal.hpp"
+#include "fstage_common.hpp"
 
 namespace gxc {
 namespace {
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 mnegi(float2 a) { return make_float2(a.y, -a.x); }  // -i*a
-
-// Multiply by the compile-time twiddle W_N^K = exp(-2 pi i K / N).
-template <int N, int K>
-__device__ __forceinline__ float2 twc(float2 a) {
-  constexpr int k = K % N;
-  constexpr float c = 0.70710678118654752f;
-  if constexpr (k == 0) {
-    return a;
-  } else if constexpr (4 * k == N) {
-    return mnegi(a);
-  } else if constexpr (2 * k == N) {
-    return make_float2(-a.x, -a.y);
-  } else if constexpr (4 * k == 3 * N) {
-    return make_float2(-a.y, a.x);
-  } else if constexpr (8 * k == N) {
-    return make_float2((a.x + a.y) * c, (a.y - a.x) * c);
-  } else if constexpr (8 * k == 3 * N) {
-    return make_float2((a.y - a.x) * c, -(a.x + a.y) * c);
-  } else if constexpr (8 * k == 5 * N) {
-    return make_float2(-(a.x + a.y) * c, (a.x - a.y) * c);
-  } else if constexpr (8 * k == 7 * N) {
-    return make_float2((a.x - a.y) * c, (a.x + a.y) * c);
-  } else {
-    static_assert(N == 16, "general twiddle constants only for N == 16");
-    constexpr float c1 = 0.92387953251128676f, s1 = 0.38268343236508977f;
-    // W16^k = cos(2 pi k/16) - i sin(2 pi k/16) for odd k
-    constexpr float wr = (k == 1 || k == 15) ? c1 : (k == 3 || k == 13) ? s1
-                         : (k == 5 || k == 11) ? -s1 : -c1;
-    constexpr float wi = (k == 1 || k == 7) ? -s1 : (k == 3 || k == 5) ? -c1
-                         : (k == 9 || k == 15) ? s1 : c1;
-    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
-  }
-}
-
-__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
-  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
-  const float2 t2 = cadd(a1, a3), t3 = mnegi(csub(a1, a3));
-  a0 = cadd(t0, t2);
-  a2 = csub(t0, t2);
-  a1 = cadd(t1, t3);
-  a3 = csub(t1, t3);
-}
-
-// In-place forward DFT of R points v[0..R), natural order in and out.
-template <int R>
-__device__ __forceinline__ void dft(float2 *v) {
-  if constexpr (R == 2) {
-    const float2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  } else if constexpr (R == 4) {
-    dft4(v[0], v[1], v[2], v[3]);
-  } else if constexpr (R == 8) {
-    // k = k1 + 2 k2: DFT4 over k2, twiddle W8^(k1 q2), DFT2 over k1.
-    dft4(v[0], v[2], v[4], v[6]);
-    dft4(v[1], v[3], v[5], v[7]);
-    const float2 y10 = v[1], y11 = twc<8, 1>(v[3]), y12 = twc<8, 2>(v[5]), y13 = twc<8, 3>(v[7]);
-    const float2 y00 = v[0], y01 = v[2], y02 = v[4], y03 = v[6];
-    v[0] = cadd(y00, y10); v[4] = csub(y00, y10);
-    v[1] = cadd(y01, y11); v[5] = csub(y01, y11);
-    v[2] = cadd(y02, y12); v[6] = csub(y02, y12);
-    v[3] = cadd(y03, y13); v[7] = csub(y03, y13);
-  } else if constexpr (R == 16) {
-    // k = k1 + 4 k2, q = 4 q1 + q2: DFT4 over k2, twiddle W16^(k1 q2), DFT4 over k1.
-    dft4(v[0], v[4], v[8], v[12]);
-    dft4(v[1], v[5], v[9], v[13]);
-    dft4(v[2], v[6], v[10], v[14]);
-    dft4(v[3], v[7], v[11], v[15]);
-    // y[k1][q2] now sits at v[k1 + 4 q2]
-    float2 y[4][4];
-#pragma unroll
-    for (int q2 = 0; q2 < 4; q2++) y[0][q2] = v[4 * q2];
-    y[1][0] = v[1]; y[1][1] = twc<16, 1>(v[5]); y[1][2] = twc<16, 2>(v[9]);  y[1][3] = twc<16, 3>(v[13]);
-    y[2][0] = v[2]; y[2][1] = twc<16, 2>(v[6]); y[2][2] = twc<16, 4>(v[10]); y[2][3] = twc<16, 6>(v[14]);
-    y[3][0] = v[3]; y[3][1] = twc<16, 3>(v[7]); y[3][2] = twc<16, 6>(v[11]); y[3][3] = twc<16, 9>(v[15]);
-#pragma unroll
-    for (int q2 = 0; q2 < 4; q2++) {
-      float2 a0 = y[0][q2], a1 = y[1][q2], a2 = y[2][q2], a3 = y[3][q2];
-      dft4(a0, a1, a2, a3);
-      v[q2] = a0; v[4 + q2] = a1; v[8 + q2] = a2; v[12 + q2] = a3;
-    }
-  }
-}
+using namespace fs;
 
 // Compile-time geometry for FFT length L.
 template <int L, bool CPLX>
 struct FG {
+  using TG = TabGeom<L, CPLX>;
   static constexpr int LG = ilog2c(L);
-  static constexpr int NCHAN = CPLX ? L : L / 2;
-  static constexpr int S = 1 << (ilog2c(NCHAN) / 2);
-  static constexpr int S2 = CPLX ? S : 2 * S;
-  static constexpr int NSTEPS = L / S2;
-  static constexpr int FRACSTEPS = NCHAN / S;
-  static constexpr int NTAB = S2 + NSTEPS + S + FRACSTEPS;
+  static constexpr int NCHAN = TG::NCHAN, S = TG::S, S2 = TG::S2, NSTEPS = TG::NSTEPS, NTAB = TG::NTAB;
   static constexpr int T = L / 16;                   // threads per item
   static constexpr int IPW = T >= 256 ? 1 : 256 / T; // items per workgroup
   static constexpr int BLOCK = T * IPW;
@@ -137,43 +45,6 @@ struct FG {
 };
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
-
-// FP64 delay model and integer/fractional split for station st, FFT t:
-// fxkernel.cpp:425-440 (getStationDelay) and 288-309, one rounding per
-// operation (no contraction), so offsets and validity match the CPU exactly.
-template <int L>
-__device__ __forceinline__ bool split_offset(const FParams &p, int st, int t, int &offset,
-                                             double &frac, double &a, double &b) {
-#pragma clang fp contract(off)
-  const double *dp = p.delays + 4 * st;
-  const double c0 = dp[0], c1 = dp[1], c2 = dp[2], fileoffset = dp[3];
-  const double t0 = (double)t, th = t + 0.5, t1 = t + 1.0;
-  const double d0 = c0 * t0 * t0 + c1 * t0 + c2;
-  const double d1 = c0 * th * th + c1 * th + c2;
-  const double d2 = c0 * t1 * t1 + c1 * t1 + c2;
-  a = d2 - d0;
-  b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
-  const double mean = a * 0.5 + b;
-  const double ndf = (mean - fileoffset) / p.sampletime;
-  if (!(fabs(ndf) < 1.0e9)) {
-    offset = 0;
-    frac = 0.0;
-    return false;
-  }
-  const int nd = (int)(ndf + 0.5);  // truncation toward zero, fxkernel.cpp:290
-  frac = -(ndf - nd) * p.sampletime;
-  long long off = (long long)t * L - nd;
-  if (off == -1) {
-    ++off;
-    frac += p.sampletime;
-  }
-  if (off == (long long)p.maxoffset + 1) {
-    --off;
-    frac -= p.sampletime;
-  }
-  offset = (int)off;
-  return !(off < 0 || off > p.maxoffset);
-}
 
 // LDS index of element i with one pad slot per 16 elements.  When a stride
 // is a multiple of 16 the padded address of i + k*stride is the padded
@@ -246,27 +117,6 @@ __device__ __forceinline__ void middle_passes(float2 (&va)[16], float2 (&vb)[16]
   }
 }
 
-__device__ __forceinline__ uint32_t pack_f16(float re, float im) {
-  const _Float16 a = (_Float16)re, b = (_Float16)im;
-  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
-}
-
-__device__ __forceinline__ uint32_t bf16_bits(float x) {
-  const uint32_t u = __builtin_bit_cast(uint32_t, x);
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even
-}
-
-__device__ __forceinline__ uint32_t pack_bf16(float re, float im) {
-  return bf16_bits(re) | (bf16_bits(im) << 16);
-}
-
-__device__ __forceinline__ uint16_t pack_fp8(float re, float im) {
-  // e4m3 with saturation to the largest finite value before conversion.
-  re = fminf(fmaxf(re, -448.f), 448.f);
-  im = fminf(fmaxf(im, -448.f), 448.f);
-  return (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false) & 0xffff);
-}
-
 template <int L, bool CPLX, int OUT>
 __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p) {
   using G = FG<L, CPLX>;
@@ -303,39 +153,7 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
   float2 *step = sub + S2;
   float2 *fsub = step + NSTEPS;
   float2 *fstep = fsub + S;
-  if (valid) {
-#pragma clang fp contract(off)
-    // fxkernel.cpp:503-505: remove whole seconds of delay.
-    const double intdelay = trunc(b);
-    const double bf = b - intdelay;
-    const float ffrac = (float)frac;  // ippsMulC_32f takes an Ipp32f (fxkernel.cpp:600)
-    for (int e = j; e < G::NTAB; e += T) {
-      float sn, cs;
-      if (e < S2) {
-        // subphase, fxkernel.cpp:107,508,514,519,525-530,535
-        double ph = ((double)e / (double)L) * a;
-        ph = ph + bf;
-        ph = ph * p.lo;
-        if (p.fraclo) ph = ph + p.lofrac * intdelay;
-        const double arg = (-2.0 * kPi) * (ph - trunc(ph));
-        sincosf((float)arg, &sn, &cs);
-      } else if (e < S2 + NSTEPS) {
-        // stepphase, fxkernel.cpp:112,511,522,539
-        const int i = e - S2;
-        double ph = ((double)(i * p.stepmul) / (double)L) * a;
-        ph = ph * p.lo;
-        const double arg = (-2.0 * kPi) * (ph - trunc(ph));
-        sincosf((float)arg, &sn, &cs);
-      } else if (e < S2 + NSTEPS + S) {
-        // fxkernel.cpp:600,608
-        sincosf(p.subchanfreq[e - S2 - NSTEPS] * ffrac, &sn, &cs);
-      } else {
-        // fxkernel.cpp:603,611
-        sincosf(p.stepchanfreq[e - S2 - NSTEPS - S] * ffrac, &sn, &cs);
-      }
-      stab[li][e] = make_float2(cs, sn);
-    }
-  }
+  if (valid) fill_tables<L, CPLX>(p, frac, a, b, stab[li], j, T);
   __syncthreads();
 
   float2 va[16], vb[16];
@@ -347,7 +165,6 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
   if (valid) {
     // Station buffers are device (global) memory; saying so lets the loads
     // below use the global rather than the flat address path.
-    typedef const __attribute__((address_space(1))) uint8_t gbyte;
     gbyte *src = (gbyte *)p.station[st];
     // Sample n = j + k*L/16 of the window is packed sample offset+n.  L/16
     // is even, so every k shares the byte parity of offset+j: one base
@@ -365,8 +182,7 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
         vb[k] = make_float2(x.y * r.x, x.y * r.y);
       } else {
         // [Are, Aim, Bre, Bim] int8, little-endian in one 32-bit word
-        const int wv = *reinterpret_cast<const __attribute__((address_space(1))) int *>(
-            bsrc + 4 * (size_t)k * (L / 16));
+        const int wv = *reinterpret_cast<gint *>(bsrc + 4 * (size_t)k * (L / 16));
         const float are = (float)(int8_t)(wv & 0xff), aim = (float)(int8_t)((wv >> 8) & 0xff);
         const float bre = (float)(int8_t)((wv >> 16) & 0xff), bim = (float)(int8_t)(wv >> 24);
         va[k] = cmul(r, make_float2(are, aim));
@@ -404,32 +220,8 @@ __global__ void __launch_bounds__((FG<L, CPLX>::BLOCK)) fstage_kernel(FParams p)
           ya = cmul(fr, va[bf * R + k]);
           yb = cmul(fr, vb[bf * R + k]);
         }
-        if constexpr (OUT == OUT_F32) {
-          p.spectra[rowA * G::NCHAN + q] = ya;
-          p.spectra[(rowA + 1) * G::NCHAN + q] = yb;
-        } else if constexpr (OUT == OUT_POWER) {
-          // one row per sampled FFT; the scale kernel sums rows in a fixed
-          // order, so the scales do not depend on scheduling
-          p.power[rowA * G::NCHAN + q] = ya.x * ya.x + ya.y * ya.y;
-          p.power[(rowA + 1) * G::NCHAN + q] = yb.x * yb.x + yb.y * yb.y;
-        } else {
-          const float sa = p.scales[(size_t)(2 * st) * G::NCHAN + q];
-          const float sb = p.scales[(size_t)(2 * st + 1) * G::NCHAN + q];
-          if constexpr (OUT == OUT_FP8) {
-            uint16_t *o = static_cast<uint16_t *>(p.out);
-            o[rowA * G::NCHAN + q] = pack_fp8(ya.x * sa, ya.y * sa);
-            o[(rowA + 1) * G::NCHAN + q] = pack_fp8(yb.x * sb, yb.y * sb);
-          } else {
-            uint32_t *o = static_cast<uint32_t *>(p.out);
-            if constexpr (OUT == OUT_F16) {
-              o[rowA * G::NCHAN + q] = pack_f16(ya.x * sa, ya.y * sa);
-              o[(rowA + 1) * G::NCHAN + q] = pack_f16(yb.x * sb, yb.y * sb);
-            } else {
-              o[rowA * G::NCHAN + q] = pack_bf16(ya.x * sa, ya.y * sa);
-              o[(rowA + 1) * G::NCHAN + q] = pack_bf16(yb.x * sb, yb.y * sb);
-            }
-          }
-        }
+        store_bin<OUT, G::NCHAN>(p, rowA, 2 * st, q, ya);
+        store_bin<OUT, G::NCHAN>(p, rowA + 1, 2 * st + 1, q, yb);
       }
     }
     if (OUT != OUT_POWER && j == 0) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
@@ -459,7 +251,7 @@ hipError_t launch_one(const FParams &p, hipStream_t s) {
 
 bool fstage_supported(int L, bool cplx) {
   (void)cplx;
-  return L >= 64 && L <= 8192 && (L & (L - 1)) == 0;
+  return L >= 64 && L <= 32768 && (L & (L - 1)) == 0;
 }
 
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s) {
@@ -476,7 +268,7 @@ hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s) {
     GXC_F_CASE(4096)
     GXC_F_CASE(8192)
     default:
-      return hipErrorInvalidValue;
+      return launch_fstage_big(p, L, cplx, s);
   }
 #undef GXC_F_CASE
 }

@@ -1,0 +1,280 @@
+// Device helpers shared by the F-stage kernels (fstage.hip: FFTs held in LDS
+// with both polarisations, L <= 8192; fstage_big.hip: L = 16384, 32768).
+#pragma once
+
+#include "gxc_internal.hpp"
+
+namespace gxc {
+namespace fs {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 mnegi(float2 a) { return make_float2(a.y, -a.x); }  // -i*a
+
+// Compile-time cos/sin for twiddle constants (Taylor series in double,
+// |x| <= 2 pi, 40 terms: accurate far below FP32 resolution).
+constexpr double cx_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 40; n++) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 40; n++) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+
+// Multiply by the compile-time twiddle W_N^K = exp(-2 pi i K / N).  Multiples
+// of pi/4 use exact forms; other angles use FP32-rounded constants.
+template <int N, int K>
+__device__ __forceinline__ float2 twc(float2 a) {
+  constexpr int k = K % N;
+  constexpr float c = 0.70710678118654752f;
+  if constexpr (k == 0) {
+    return a;
+  } else if constexpr (4 * k == N) {
+    return mnegi(a);
+  } else if constexpr (2 * k == N) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (4 * k == 3 * N) {
+    return make_float2(-a.y, a.x);
+  } else if constexpr (8 * k == N) {
+    return make_float2((a.x + a.y) * c, (a.y - a.x) * c);
+  } else if constexpr (8 * k == 3 * N) {
+    return make_float2((a.y - a.x) * c, -(a.x + a.y) * c);
+  } else if constexpr (8 * k == 5 * N) {
+    return make_float2(-(a.x + a.y) * c, (a.x - a.y) * c);
+  } else if constexpr (8 * k == 7 * N) {
+    return make_float2((a.x - a.y) * c, (a.x + a.y) * c);
+  } else {
+    constexpr double ang = 2.0 * kPi * (double)k / (double)N;
+    constexpr float wr = (float)cx_cos(ang), wi = (float)-cx_sin(ang);
+    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+  }
+}
+
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  const float2 t2 = cadd(a1, a3), t3 = mnegi(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int K>
+__device__ __forceinline__ float2 tw32(float2 a) {
+  return twc<32, K>(a);
+}
+
+// In-place forward DFT of R points v[0..R), natural order in and out.
+template <int R>
+__device__ __forceinline__ void dft(float2 *v) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    dft4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    // k = k1 + 2 k2: DFT4 over k2, twiddle W8^(k1 q2), DFT2 over k1.
+    dft4(v[0], v[2], v[4], v[6]);
+    dft4(v[1], v[3], v[5], v[7]);
+    const float2 y10 = v[1], y11 = twc<8, 1>(v[3]), y12 = twc<8, 2>(v[5]), y13 = twc<8, 3>(v[7]);
+    const float2 y00 = v[0], y01 = v[2], y02 = v[4], y03 = v[6];
+    v[0] = cadd(y00, y10); v[4] = csub(y00, y10);
+    v[1] = cadd(y01, y11); v[5] = csub(y01, y11);
+    v[2] = cadd(y02, y12); v[6] = csub(y02, y12);
+    v[3] = cadd(y03, y13); v[7] = csub(y03, y13);
+  } else if constexpr (R == 16) {
+    // k = k1 + 4 k2, q = 4 q1 + q2: DFT4 over k2, twiddle W16^(k1 q2), DFT4 over k1.
+    dft4(v[0], v[4], v[8], v[12]);
+    dft4(v[1], v[5], v[9], v[13]);
+    dft4(v[2], v[6], v[10], v[14]);
+    dft4(v[3], v[7], v[11], v[15]);
+    float2 y[4][4];
+#pragma unroll
+    for (int q2 = 0; q2 < 4; q2++) y[0][q2] = v[4 * q2];
+    y[1][0] = v[1]; y[1][1] = twc<16, 1>(v[5]); y[1][2] = twc<16, 2>(v[9]);  y[1][3] = twc<16, 3>(v[13]);
+    y[2][0] = v[2]; y[2][1] = twc<16, 2>(v[6]); y[2][2] = twc<16, 4>(v[10]); y[2][3] = twc<16, 6>(v[14]);
+    y[3][0] = v[3]; y[3][1] = twc<16, 3>(v[7]); y[3][2] = twc<16, 6>(v[11]); y[3][3] = twc<16, 9>(v[15]);
+#pragma unroll
+    for (int q2 = 0; q2 < 4; q2++) {
+      float2 a0 = y[0][q2], a1 = y[1][q2], a2 = y[2][q2], a3 = y[3][q2];
+      dft4(a0, a1, a2, a3);
+      v[q2] = a0; v[4 + q2] = a1; v[8 + q2] = a2; v[12 + q2] = a3;
+    }
+  } else if constexpr (R == 32) {
+    // Even/odd split: X[q] = E[q] + W32^q O[q], X[q+16] = E[q] - W32^q O[q].
+    float2 e[16], o[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      e[m] = v[2 * m];
+      o[m] = v[2 * m + 1];
+    }
+    dft<16>(e);
+    dft<16>(o);
+    const float2 w[16] = {o[0],         tw32<1>(o[1]),   tw32<2>(o[2]),   tw32<3>(o[3]),
+                          tw32<4>(o[4]), tw32<5>(o[5]),   tw32<6>(o[6]),   tw32<7>(o[7]),
+                          tw32<8>(o[8]), tw32<9>(o[9]),   tw32<10>(o[10]), tw32<11>(o[11]),
+                          tw32<12>(o[12]), tw32<13>(o[13]), tw32<14>(o[14]), tw32<15>(o[15])};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      v[q] = cadd(e[q], w[q]);
+      v[q + 16] = csub(e[q], w[q]);
+    }
+  }
+}
+
+// Sizes of the per-item rotator tables for FFT length L (fxkernel.cpp:96-114):
+// sub and step fringe-rotator factors, then sub and step fractional-sample
+// factors.
+template <int L, bool CPLX>
+struct TabGeom {
+  static constexpr int NCHAN = CPLX ? L : L / 2;
+  static constexpr int S = 1 << (ilog2c(NCHAN) / 2);
+  static constexpr int S2 = CPLX ? S : 2 * S;
+  static constexpr int NSTEPS = L / S2;
+  static constexpr int FRACSTEPS = NCHAN / S;
+  static constexpr int NTAB = S2 + NSTEPS + S + FRACSTEPS;
+};
+
+// FP64 delay model and integer/fractional split for station st, FFT t:
+// fxkernel.cpp:425-440 (getStationDelay) and 288-309, one rounding per
+// operation (no contraction), so offsets and validity match the CPU exactly.
+template <int L>
+__device__ __forceinline__ bool split_offset(const FParams &p, int st, int t, int &offset,
+                                             double &frac, double &a, double &b) {
+#pragma clang fp contract(off)
+  const double *dp = p.delays + 4 * st;
+  const double c0 = dp[0], c1 = dp[1], c2 = dp[2], fileoffset = dp[3];
+  const double t0 = (double)t, th = t + 0.5, t1 = t + 1.0;
+  const double d0 = c0 * t0 * t0 + c1 * t0 + c2;
+  const double d1 = c0 * th * th + c1 * th + c2;
+  const double d2 = c0 * t1 * t1 + c1 * t1 + c2;
+  a = d2 - d0;
+  b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+  const double mean = a * 0.5 + b;
+  const double ndf = (mean - fileoffset) / p.sampletime;
+  if (!(fabs(ndf) < 1.0e9)) {
+    offset = 0;
+    frac = 0.0;
+    return false;
+  }
+  const int nd = (int)(ndf + 0.5);  // truncation toward zero, fxkernel.cpp:290
+  frac = -(ndf - nd) * p.sampletime;
+  long long off = (long long)t * L - nd;
+  if (off == -1) {
+    ++off;
+    frac += p.sampletime;
+  }
+  if (off == (long long)p.maxoffset + 1) {
+    --off;
+    frac -= p.sampletime;
+  }
+  offset = (int)off;
+  return !(off < 0 || off > p.maxoffset);
+}
+
+// Entries e = e0, e0+estep, ... of one item's rotator tables: sub, step
+// (fringe rotation, fxkernel.cpp:498-567) and fsub, fstep (fractional-sample
+// correction, fxkernel.cpp:595-634).  Phases in FP64, reduced, rounded to
+// FP32, then sincosf.
+template <int L, bool CPLX>
+__device__ __forceinline__ void fill_tables(const FParams &p, double frac, double a, double b, float2 *tab,
+                                            int e0, int estep) {
+#pragma clang fp contract(off)
+  using TG = TabGeom<L, CPLX>;
+  constexpr int S = TG::S, S2 = TG::S2, NSTEPS = TG::NSTEPS;
+  // fxkernel.cpp:503-505: remove whole seconds of delay.
+  const double intdelay = trunc(b);
+  const double bf = b - intdelay;
+  const float ffrac = (float)frac;  // ippsMulC_32f takes an Ipp32f (fxkernel.cpp:600)
+  for (int e = e0; e < TG::NTAB; e += estep) {
+    float sn, cs;
+    if (e < S2) {
+      // subphase, fxkernel.cpp:107,508,514,519,525-530,535
+      double ph = ((double)e / (double)L) * a;
+      ph = ph + bf;
+      ph = ph * p.lo;
+      if (p.fraclo) ph = ph + p.lofrac * intdelay;
+      const double arg = (-2.0 * kPi) * (ph - trunc(ph));
+      sincosf((float)arg, &sn, &cs);
+    } else if (e < S2 + NSTEPS) {
+      // stepphase, fxkernel.cpp:112,511,522,539
+      const int i = e - S2;
+      double ph = ((double)(i * p.stepmul) / (double)L) * a;
+      ph = ph * p.lo;
+      const double arg = (-2.0 * kPi) * (ph - trunc(ph));
+      sincosf((float)arg, &sn, &cs);
+    } else if (e < S2 + NSTEPS + S) {
+      // fxkernel.cpp:600,608
+      sincosf(p.subchanfreq[e - S2 - NSTEPS] * ffrac, &sn, &cs);
+    } else {
+      // fxkernel.cpp:603,611
+      sincosf(p.stepchanfreq[e - S2 - NSTEPS - S] * ffrac, &sn, &cs);
+    }
+    tab[e] = make_float2(cs, sn);
+  }
+}
+
+__device__ __forceinline__ uint32_t pack_f16(float re, float im) {
+  const _Float16 a = (_Float16)re, b = (_Float16)im;
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(float re, float im) {
+  return bf16_bits(re) | (bf16_bits(im) << 16);
+}
+
+__device__ __forceinline__ uint16_t pack_fp8(float re, float im) {
+  // e4m3 with saturation to the largest finite value before conversion.
+  re = fminf(fmaxf(re, -448.f), 448.f);
+  im = fminf(fmaxf(im, -448.f), 448.f);
+  return (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false) & 0xffff);
+}
+
+// Store bin q of output row `row` (= 2*station + pol within FFT row block)
+// in the spectra format OUT.  `srow` is the input index 2*station+pol.
+template <int OUT, int NCHAN>
+__device__ __forceinline__ void store_bin(const FParams &p, size_t row, int srow, int q, float2 y) {
+  if constexpr (OUT == OUT_F32) {
+    p.spectra[row * NCHAN + q] = y;
+  } else if constexpr (OUT == OUT_POWER) {
+    // one row per sampled FFT; the scale kernel sums rows in a fixed order,
+    // so the scales do not depend on scheduling
+    p.power[row * NCHAN + q] = y.x * y.x + y.y * y.y;
+  } else {
+    const float sc = p.scales[(size_t)srow * NCHAN + q];
+    if constexpr (OUT == OUT_FP8) {
+      static_cast<uint16_t *>(p.out)[row * NCHAN + q] = pack_fp8(y.x * sc, y.y * sc);
+    } else if constexpr (OUT == OUT_F16) {
+      static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_f16(y.x * sc, y.y * sc);
+    } else {
+      static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_bf16(y.x * sc, y.y * sc);
+    }
+  }
+}
+
+// Global-address-space view of the packed station buffers: the loads then
+// use the global rather than the flat address path.
+typedef const __attribute__((address_space(1))) uint8_t gbyte;
+typedef const __attribute__((address_space(1))) int gint;
+
+}  // namespace fs
+}  // namespace gxc

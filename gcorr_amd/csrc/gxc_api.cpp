@@ -79,7 +79,7 @@ gxc_status derive(const gxc_config *c, Derived *d, std::string *why) {
   d->L = d->g.L;
   d->M = 2 * c->nant;
   if (!gxc::fstage_supported(d->L, c->is_complex != 0)) {
-    *why = "FFT length " + std::to_string(d->L) + " not supported (64..8192)";
+    *why = "FFT length " + std::to_string(d->L) + " not supported (powers of two 64..32768)";
     return GXC_ERR_UNSUPPORTED;
   }
   if (gxc::xm_blocks(d->M) == 0) {
@@ -619,8 +619,11 @@ gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station) {
 gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station) {
   if (!h || !per_station) return set_err(h, GXC_ERR_INVALID_ARG, "null input");
   std::vector<const void *> in(per_station, per_station + h->cfg.nant);
-  for (const void *p : in)
+  for (const void *p : in) {
     if (!p) return set_err(h, GXC_ERR_INVALID_ARG, "null station buffer");
+    if (reinterpret_cast<uintptr_t>(p) % 4)
+      return set_err(h, GXC_ERR_INVALID_ARG, "device station buffers must be 4-byte aligned");
+  }
   GXC_HIP(h, hipStreamSynchronize(h->stream));
   h->dev_in = in;
   GXC_HIP(h, hipMemcpy(h->d_station_ptrs, h->dev_in.data(), h->cfg.nant * sizeof(void *), hipMemcpyHostToDevice));

@@ -67,7 +67,8 @@ struct CountParams {
 
 // Launchers (defined in the .hip files).  Return hipSuccess or the launch error.
 bool fstage_supported(int L, bool cplx);
-hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s);
+hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s);      // any supported L
+hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s);  // L = 16384, 32768
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
 
 }  // namespace gxc

@@ -49,7 +49,7 @@ def test_create_without_device_or_bad_config_fails_cleanly():
     assert create(nbit=4) == 2            # fxkernel supports 2-bit real only
     assert create(npol=1) == 2
     assert create(is_complex=1) == 2      # complex must be 8-bit
-    assert create(nchan=16384) == 2       # FFT length 32768 needs the large-FFT path
+    assert create(nchan=32768) == 2       # real input: FFT length 65536 is beyond 32768
     assert create(precision=7) == 1
 
 

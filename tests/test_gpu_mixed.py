@@ -117,3 +117,15 @@ def test_mixed_is_deterministic(gpu):
         out.append(k.visibilities())
         k.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp8"])
+def test_c5_shape_vs_fp64_oracle(gpu, precision):
+    """32 stations x 2 pols of complex int8, 32768-point FFTs (the largest
+    workload's shape: 64 inputs, 4 blocks of 16 in the X-engine), 24 of its
+    15 625 FFTs; FP8 e4m3 against the FP16 and FP32 forms and the FP64 oracle."""
+    w = synth.WORKLOADS["c5"].scaled(24)
+    v, ref = _mixed_vs_oracle(w, precision)
+    e = metrics.eps(v, ref)
+    print(precision, "eps", e, metrics.amp_phase_errors(v, ref))
+    assert e < {"fp32": 1e-5, "fp16": 1e-3, "fp8": BOUND["fp8"]}[precision]

@@ -56,7 +56,8 @@ def run_oracle(w, data, poly, offs, autos=True, mode="f32", quirk=True):
 @pytest.mark.parametrize("nchan,is_complex", [(32, False), (64, False), (128, False), (256, False),
                                               (512, False), (1024, False), (2048, False),
                                               (4096, False), (256, True), (1024, True),
-                                              (8192, True)])
+                                              (8192, True), (8192, False), (16384, False),
+                                              (16384, True), (32768, True)])
 def test_fstage_matches_oracle(gpu, nchan, is_complex):
     w, data, poly, offs = random_case(3, nchan, 40, seed=nchan + is_complex, is_complex=is_complex,
                                       bw=128e6 if is_complex else 64e6)
