@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp8"])
     ap.add_argument("--numffts", type=int, default=0, help="override FFTs per subint (testing)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -100,15 +100,22 @@ def main():
     import gcorr_amd
     from gcorr_amd import synth
 
-    w = synth.WORKLOADS[args.workload]
-    if args.numffts:
-        w = w.scaled(args.numffts)
+    if args.workload == "c4":
+        ws = synth.c4_subbands()
+        if args.numffts:  # keep the subbands' 2:1 FFT-count pattern
+            top = max(x.numffts for x in ws)
+            ws = [x.scaled(max(8, x.numffts * args.numffts // top)) for x in ws]
+    else:
+        ws = [synth.WORKLOADS[args.workload]]
+        if args.numffts:
+            ws = [ws[0].scaled(args.numffts)]
+    w = ws[0]
     dev = torch.device("cuda:0")
     t = time.time()
-    bufs, poly, offs = synth.generate(w, device=dev)
+    gens = [synth.generate(x, device=dev) for x in ws]
     torch.cuda.synchronize()
     gen_s = time.time() - t
-    print("generated %s in %.1f s" % (w.name, gen_s), file=sys.stderr, flush=True)
+    print("generated %s in %.1f s" % (args.workload, gen_s), file=sys.stderr, flush=True)
 
     if args.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
@@ -116,11 +123,20 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
-                           precision=args.precision, autos=True, overlap=not args.no_overlap,
-                           chunk_ffts=args.chunk_ffts)
-    k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
-    k.setDelays(poly, offs)
+    if args.workload == "c4":
+        k = gcorr_amd.Batch(w.nant, [(x.nchan, x.numffts, x.lo, x.bw) for x in ws],
+                            precision=args.precision, autos=True, overlap=not args.no_overlap)
+        for sidx, (bufs, poly, offs) in enumerate(gens):
+            k.set_input_device(sidx, [b.data_ptr() for b in bufs], keepalive=bufs)
+            k.set_delays(sidx, poly, offs)
+    else:
+        bufs, poly, offs = gens[0]
+        k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                               precision=args.precision, autos=True, overlap=not args.no_overlap,
+                               chunk_ffts=args.chunk_ffts)
+        k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
+        k.setDelays(poly, offs)
+    input_samples = sum(x.input_samples for x in ws)
     for _ in range(args.warmup):
         k.process()
     k.set_profiling(True)
@@ -149,7 +165,7 @@ def main():
         k.process()
     torch.cuda.synchronize()
     graph_ms_per_step = (time.perf_counter() - t) * 1e3 / args.steps
-    value = w.input_samples / (ms_per_step * 1e-3) / 1e9
+    value = input_samples / (ms_per_step * 1e-3) / 1e9
 
     # algorithmic bytes per launch: input bits + nchan-bin spectra (F);
     # spectra read once + visibilities written (X)
@@ -158,9 +174,12 @@ def main():
     if w.is_complex:
         spec_bytes_per_sample *= 2
     nrows = gcorr_amd.num_rows(w.nant, autos=True)
-    vis_bytes = nrows * 4 * w.nchan * 8
-    f_bytes = w.input_samples * (in_bytes_per_sample + spec_bytes_per_sample) / max(1, f_launch / args.steps)
-    x_bytes = (w.input_samples * spec_bytes_per_sample) / max(1, x_launch / args.steps) + vis_bytes
+    f_per_step = max(1, f_launch / args.steps)
+    x_per_step = max(1, x_launch / args.steps)
+    f_bytes = input_samples * (in_bytes_per_sample + spec_bytes_per_sample) / f_per_step
+    # X: the spectra read once, the visibilities written once, per subint
+    vis_bytes = sum(nrows * 4 * x.nchan * 8 for x in ws)
+    x_bytes = (input_samples * spec_bytes_per_sample + vis_bytes) / x_per_step
     f_avg = f_ms / max(1, f_launch)
     x_avg = x_ms / max(1, x_launch)
     stages = {
@@ -186,7 +205,7 @@ def main():
     }
 
     h2d = None
-    if not args.no_h2d:
+    if not args.no_h2d and args.workload != "c4":
         host = [b.cpu().numpy() for b in bufs]
         kh = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
                                 precision=args.precision, autos=True)
@@ -202,7 +221,7 @@ def main():
         del host
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload != "c4":
         cpu = cpu_baseline(w, poly, offs, args.cpu_seconds)
 
     out = {
@@ -219,13 +238,16 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else args.precision,
         "data": "synthetic (seeded signal model generated in HBM: common signal, quadratic delays, fringes, 2-bit quantised)",
         "config": {
-            "workload": w.name,
-            "stations": w.nant, "pols": 2, "nchan": w.nchan, "fft": w.fftchannels,
-            "numffts": w.numffts, "bandwidth_hz": w.bw, "lo_hz": w.lo,
+            "workload": w.name if len(ws) == 1 else "8st_2pol_8subbands_16-128MHz_256-4096ch_0.5s",
+            "stations": w.nant, "pols": 2,
+            "nchan": [x.nchan for x in ws] if len(ws) > 1 else w.nchan,
+            "fft": [x.fftchannels for x in ws] if len(ws) > 1 else w.fftchannels,
+            "numffts": [x.numffts for x in ws] if len(ws) > 1 else w.numffts,
+            "bandwidth_hz": [x.bw for x in ws] if len(ws) > 1 else w.bw, "lo_hz": w.lo,
             "input": "complex int8" if w.is_complex else "real 2-bit",
             "precision": args.precision, "autos": True,
             "overlap": not args.no_overlap, "chunk_ffts": args.chunk_ffts or "auto",
-            "input_samples_per_step": w.input_samples,
+            "input_samples_per_step": input_samples,
         },
         "roofline": roofline,
         "stages": stages,

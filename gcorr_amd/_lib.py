@@ -52,6 +52,21 @@ class ConfFile(ctypes.Structure):
     ]
 
 
+class Subband(ctypes.Structure):
+    _fields_ = [("nchan", ctypes.c_int), ("numffts", ctypes.c_int),
+                ("lo", ctypes.c_double), ("bw", ctypes.c_double)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("nant", ctypes.c_int), ("nbit", ctypes.c_int), ("npol", ctypes.c_int),
+        ("is_complex", ctypes.c_int), ("precision", ctypes.c_int), ("autos", ctypes.c_int),
+        ("exact_phase", ctypes.c_int), ("nsub", ctypes.c_int),
+        ("subbands", ctypes.POINTER(Subband)), ("chunks", ctypes.c_int),
+        ("overlap", ctypes.c_int), ("graph", ctypes.c_int), ("reserved", ctypes.c_int * 4),
+    ]
+
+
 class CF32(ctypes.Structure):
     _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
 
@@ -104,6 +119,20 @@ def load():
         "gxc_get_timings": (ctypes.c_int, [H, P(Timings)]),
         "gxc_parse_config": (ctypes.c_int, [ctypes.c_char_p, P(ConfFile)]),
         "gxc_conffile_free": (None, [P(ConfFile)]),
+        "gxc_batch_config_init": (None, [P(BatchConfig)]),
+        "gxc_batch_create": (ctypes.c_int, [P(BatchConfig), P(vp)]),
+        "gxc_batch_destroy": (None, [H]),
+        "gxc_batch_last_error": (ctypes.c_char_p, [H]),
+        "gxc_batch_set_input": (ctypes.c_int, [H, ctypes.c_int, P(u8p)]),
+        "gxc_batch_set_input_device": (ctypes.c_int, [H, ctypes.c_int, P(vp)]),
+        "gxc_batch_set_delays": (ctypes.c_int, [H, ctypes.c_int, P(dp), dp]),
+        "gxc_batch_process": (ctypes.c_int, [H]),
+        "gxc_batch_process_async": (ctypes.c_int, [H]),
+        "gxc_batch_synchronize": (ctypes.c_int, [H]),
+        "gxc_batch_get_visibilities": (ctypes.c_int, [H, ctypes.c_int, vp, ctypes.c_size_t]),
+        "gxc_batch_get_row_counts": (ctypes.c_int, [H, ctypes.c_int, ip, ctypes.c_size_t]),
+        "gxc_batch_set_profiling": (ctypes.c_int, [H, ctypes.c_int]),
+        "gxc_batch_get_timings": (ctypes.c_int, [H, P(Timings)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -116,4 +145,10 @@ def load():
 def check(status, handle=None):
     if status != GXC_OK:
         msg = load().gxc_last_error(handle)
+        raise GxcError(status, msg.decode() if msg else "")
+
+
+def check_batch(status, handle=None):
+    if status != GXC_OK:
+        msg = load().gxc_batch_last_error(handle)
         raise GxcError(status, msg.decode() if msg else "")

@@ -4,13 +4,16 @@
 // window lay inside its buffer (fxkernel.cpp:304-309).  A row (station pair,
 // or a station with itself for autocorrelations) counts the FFTs in which
 // both stations were valid: fxkernel's baselineCount.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "gxc_internal.hpp"
 
 namespace gxc {
 namespace {
 
-__global__ void __launch_bounds__(256) counts_kernel(CountParams p) {
-  const int row = blockIdx.x;
+__device__ __forceinline__ void count_row(const CountParams &p, int row) {
   const int2 jk = p.rowmap[row];
   const uint8_t *vj = p.valid + (size_t)jk.x * p.numffts;
   const uint8_t *vk = p.valid + (size_t)jk.y * p.numffts;
@@ -24,9 +27,55 @@ __global__ void __launch_bounds__(256) counts_kernel(CountParams p) {
     __syncthreads();
   }
   if (threadIdx.x == 0) p.counts[row] = red[0];
+  __syncthreads();  // red is reused by the next row of a persistent launch
+}
+
+__global__ void __launch_bounds__(256) counts_kernel(CountParams p) { count_row(p, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) counts_multi_kernel(const CountSeg *segs, int nseg, long long total) {
+  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+    int k = 0;
+    while (k + 1 < nseg && u >= segs[k + 1].unit0) ++k;
+    count_row(segs[k].p, (int)(u - segs[k].unit0));
+  }
 }
 
 }  // namespace
+
+long long persistent_grid(const void *kernel, int block, size_t dyn_lds, long long total) {
+  // (device, kernel, block, LDS) -> workgroups resident on the whole device
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void *, int, size_t>, long long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  const auto key = std::make_tuple(dev, kernel, block, dyn_lds);
+  long long g = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) g = it->second;
+  }
+  if (g == 0) {
+    int ncu = 0, per_cu = 0;
+    if (dev < 0 || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+      ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    g = (long long)ncu * per_cu;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = g;
+  }
+  return total < g ? total : g;
+}
+
+hipError_t launch_counts_multi(const CountSeg *segs, int nseg, long long total, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const long long grid = persistent_grid(reinterpret_cast<const void *>(&counts_multi_kernel), 256, 0, total);
+  hipLaunchKernelGGL(counts_multi_kernel, dim3((unsigned)grid), dim3(256), 0, s, segs, nseg, total);
+  return hipGetLastError();
+}
 
 hipError_t launch_counts(const CountParams &p, hipStream_t s) {
   if (p.nrows <= 0) return hipSuccess;

@@ -23,88 +23,19 @@
 #include <string>
 #include <vector>
 
+#include "gxc_host.hpp"
 #include "gxc_internal.hpp"
 #include "gxc_xengine.hpp"
 
 using gxc::Geometry;
+using gxc::host::Derived;
+using gxc::host::derive;
+using gxc::host::make_rowmap;
+using gxc::host::out_mode_of;
 
 namespace {
 
 thread_local std::string g_thread_error;
-
-struct Derived {
-  Geometry g;
-  int L = 0, M = 0;
-  int maxoffset = 0, nbl = 0, nrows = 0;
-  double sampletime = 0;
-  size_t station_bytes = 0;
-};
-
-gxc_status derive(const gxc_config *c, Derived *d, std::string *why) {
-  if (!c) {
-    *why = "null config";
-    return GXC_ERR_INVALID_ARG;
-  }
-  if (c->nant < 1) {
-    *why = "nant must be >= 1";
-    return GXC_ERR_INVALID_ARG;
-  }
-  if (c->numffts < 1) {
-    *why = "numffts must be >= 1";
-    return GXC_ERR_INVALID_ARG;
-  }
-  if (c->nchan < 2 || (c->nchan & (c->nchan - 1))) {
-    *why = "nchan must be a power of two";
-    return GXC_ERR_INVALID_ARG;
-  }
-  if (!(c->bw > 0)) {
-    *why = "bandwidth must be positive";
-    return GXC_ERR_INVALID_ARG;
-  }
-  if (c->npol != 2) {
-    *why = "only dual-polarisation data is supported (npol = 2)";
-    return GXC_ERR_UNSUPPORTED;
-  }
-  if (c->is_complex ? c->nbit != 8 : c->nbit != 2) {
-    // fxkernel.cpp:46-55,489-496: 2-bit real is the only supported input;
-    // complex 8-bit is this build's extension.
-    *why = c->is_complex ? "complex data must be 8-bit" : "real data must be 2-bit";
-    return GXC_ERR_UNSUPPORTED;
-  }
-  if (c->precision < GXC_PREC_FP32 || c->precision > GXC_PREC_FP8) {
-    *why = "unknown precision";
-    return GXC_ERR_INVALID_ARG;
-  }
-  d->g = Geometry::make(c->nchan, c->is_complex != 0);
-  d->L = d->g.L;
-  d->M = 2 * c->nant;
-  if (!gxc::fstage_supported(d->L, c->is_complex != 0)) {
-    *why = "FFT length " + std::to_string(d->L) + " not supported (powers of two 64..32768)";
-    return GXC_ERR_UNSUPPORTED;
-  }
-  if (gxc::xm_blocks(d->M) == 0) {
-    *why = "at most 32 stations (64 inputs) per handle";
-    return GXC_ERR_UNSUPPORTED;
-  }
-  if ((long long)c->numffts * d->L + d->L > 0x7fffffffLL) {
-    *why = "subintegration too long for 32-bit sample offsets";
-    return GXC_ERR_UNSUPPORTED;
-  }
-  d->maxoffset = (c->numffts - 1) * d->L;  // fxkernel.cpp:272
-  d->nbl = c->nant * (c->nant - 1) / 2;
-  d->nrows = d->nbl + (c->autos ? c->nant : 0);
-  // fxkernel.cpp:31,37-38
-  d->sampletime = 1.0 / (2.0 * c->bw);
-  if (c->is_complex) d->sampletime *= 2;
-  const size_t samples = (size_t)(c->numffts + 1) * (size_t)d->L;  // common.cpp:152
-  d->station_bytes = c->is_complex ? 4 * samples : samples / 2;
-  return GXC_OK;
-}
-
-int out_mode_of(int precision) {
-  static const int mode_of[4] = {gxc::OUT_F32, gxc::OUT_F16, gxc::OUT_BF16, gxc::OUT_FP8};
-  return mode_of[precision];
-}
 
 }  // namespace
 
@@ -184,16 +115,9 @@ gxc_status set_err(gxc_handle *h, gxc_status s, const std::string &msg) {
   } while (0)
 
 gxc_status check_device(gxc_handle *h, int *dev) {
-  int n = 0;
-  hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n < 1) return set_err(h, GXC_ERR_NO_DEVICE, "no HIP device available");
-  GXC_HIP(h, hipGetDevice(dev));
-  hipDeviceProp_t prop;
-  GXC_HIP(h, hipGetDeviceProperties(&prop, *dev));
-  if (!strstr(prop.gcnArchName, "gfx950"))
-    return set_err(h, GXC_ERR_NO_DEVICE,
-                   std::string("library is built for gfx950, device is ") + prop.gcnArchName);
-  return GXC_OK;
+  std::string why;
+  const gxc_status st = gxc::host::check_device(dev, &why);
+  return st == GXC_OK ? GXC_OK : set_err(h, st, why);
 }
 
 template <typename T>
@@ -208,36 +132,11 @@ gxc_status dmalloc(gxc_handle *h, T **p, size_t nelem) {
   return GXC_OK;
 }
 
-// Rows: cross baselines in fxkernel order (fxkernel.cpp:328-355), then autos.
-std::vector<int2> make_rowmap(int nant, bool autos) {
-  std::vector<int2> rm;
-  for (int j = 0; j < nant - 1; j++)
-    for (int k = j + 1; k < nant; k++) rm.push_back(make_int2(j, k));
-  if (autos)
-    for (int a = 0; a < nant; a++) rm.push_back(make_int2(a, a));
-  return rm;
-}
-
 gxc_status upload_tables(gxc_handle *h) {
-  const Derived &d = h->d;
-  const int L = d.L, s = d.g.s, nsf = d.g.fracsteps, nchan = h->cfg.nchan;
-  const double bw = h->cfg.bw;
-  std::vector<float2> tw(L);
-  for (int m = 0; m < L; m++) {
-    const double ang = -2.0 * gxc::kPi * (double)m / (double)L;
-    tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-  }
-  // fxkernel.cpp:171-175 channel-frequency tables, identical expressions.
-  std::vector<float> subf(s), stepf(nsf);
-  for (int i = 0; i < s; i++) subf[i] = (float)((2 * gxc::kPi * i * bw) / (double)nchan);
-  for (int i = 0; i < nsf; i++) stepf[i] = (float)((2 * gxc::kPi * i * s * bw) / (double)nchan);
-  GXC_TRY(dmalloc(h, &h->d_twiddle, L));
-  GXC_TRY(dmalloc(h, &h->d_subfreq, s));
-  GXC_TRY(dmalloc(h, &h->d_stepfreq, nsf));
-  GXC_HIP(h, hipMemcpy(h->d_twiddle, tw.data(), L * sizeof(float2), hipMemcpyHostToDevice));
-  GXC_HIP(h, hipMemcpy(h->d_subfreq, subf.data(), s * sizeof(float), hipMemcpyHostToDevice));
-  GXC_HIP(h, hipMemcpy(h->d_stepfreq, stepf.data(), nsf * sizeof(float), hipMemcpyHostToDevice));
-  return GXC_OK;
+  std::string why;
+  const gxc_status st = gxc::host::upload_tables(h->d, h->cfg.bw, h->cfg.nchan, &h->d_twiddle, &h->d_subfreq,
+                                                 &h->d_stepfreq, &why);
+  return st == GXC_OK ? GXC_OK : set_err(h, st, why);
 }
 
 void mark(gxc_handle *h, int kind, hipStream_t s) {

@@ -65,10 +65,35 @@ struct CountParams {
   int numffts, nrows;
 };
 
+// One subband's share of a multi-subband launch (units of segment k are
+// unit0[k] .. unit0[k+1]-1, segments in increasing unit0).
+struct FSeg {  // units = workgroups of the multi F launch for this subband
+  FParams p;
+  int L;
+  long long unit0;
+};
+struct CountSeg {  // units = nrows
+  CountParams p;
+  long long unit0;
+};
+
+// Workgroups that keep every CU busy for a persistent launch of `kernel`
+// (occupancy x CU count), at most `total`.
+long long persistent_grid(const void *kernel, int block, size_t dyn_lds, long long total);
+
 // Launchers (defined in the .hip files).  Return hipSuccess or the launch error.
 bool fstage_supported(int L, bool cplx);
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s);      // any supported L
 hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s);  // L = 16384, 32768
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
+// Multi-subband F-stage: real 2-bit input, FFT lengths 256..8192, workgroups
+// of `block` threads (256, or 512 when a subband has L = 8192); units of a
+// subband = ceil(nant * nt / (block / (L/16))).
+bool fstage_multi_supported(int L, bool cplx);
+int fstage_multi_block(int maxL);
+long long fstage_multi_units(int L, int block, long long items);
+hipError_t launch_fstage_multi(const FSeg *segs, int nseg, long long total, int out_mode, int block,
+                               hipStream_t s);
+hipError_t launch_counts_multi(const CountSeg *segs, int nseg, long long total, hipStream_t s);
 
 }  // namespace gxc

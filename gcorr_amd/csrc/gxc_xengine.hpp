@@ -27,6 +27,24 @@ struct FinMParams {
   int M, nchan, nrows, splits;
 };
 
+// One subband's share of a multi-subband launch.  Units of segment k are
+// unit0[k] .. unit0[k+1]-1; segments are in increasing unit0.
+struct XSeg {  // units = (nchan / channel tile) * splits
+  XMParams p;
+  long long unit0;
+};
+struct FinSeg {  // units = nrows * 4 * ceil(nchan / 256)
+  FinMParams p;
+  long long unit0;
+};
+struct ScaleSeg {  // units = ceil(n / 256)
+  float *scales;
+  const float *power;
+  int n, nrows;
+  float target;
+  long long unit0;
+};
+
 // Inputs per MFMA block (rows of a 16x16x32 tile) and channels per workgroup.
 int xm_blocks(int M);                 // ceil(M/16) in {1, 2, 4}; 0 if unsupported
 int xm_channels_per_group(int M, int out_mode);  // workgroup channel tile
@@ -43,6 +61,11 @@ hipError_t launch_finalise_mixed(const FinMParams &p, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 // Pack FP32 complex spectra [n] with unit scale into the out_mode format.
 hipError_t launch_quantise(void *out, const float2 *in, size_t n, int out_mode, hipStream_t s);
+// Persistent launches over device-resident segment tables (all subbands of
+// one X launch share M).
+hipError_t launch_xmfma_multi(const XSeg *segs, int nseg, long long total, int out_mode, int M, hipStream_t s);
+hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, hipStream_t s);
+hipError_t launch_scales_multi(const ScaleSeg *segs, int nseg, long long total, hipStream_t s);
 // RMS that the per-(input, channel) scales normalise the spectra to.
 float xm_target_rms(int out_mode);
 

@@ -98,15 +98,16 @@ __device__ __forceinline__ void stage_store(unsigned char *lds, const uint4 &v, 
   }
 }
 
+// Channels group*CT .. group*CT+CT-1, FFT split `split`, on a workgroup of
+// 256 threads (every thread calls it with the same arguments: it contains
+// __syncthreads).  lds holds XM::LDSB bytes.
 template <int ET, int MB>
-__global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
+__device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int split, unsigned char *lds) {
   using X = XM<ET, MB>;
   using Op = typename Operand<ET>::type;
   constexpr int NBATCH = 8;  // 16-byte loads issued before their LDS stores
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c0 = blockIdx.x * X::CT;
-  const int split = blockIdx.y;
+  const int c0 = group * X::CT;
   const int tb = (int)((long long)p.nt * split / p.splits);
   const int te = (int)((long long)p.nt * (split + 1) / p.splits);
   const int M = p.M;
@@ -214,6 +215,34 @@ __global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
   }
 }
 
+template <int ET, int MB>
+__global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  xmfma_item<ET, MB>(p, blockIdx.x, blockIdx.y, lds);
+}
+
+// Segment of a multi-subband launch holding unit u (segments in order).
+template <typename Seg>
+__device__ __forceinline__ int find_seg(const Seg *segs, int nseg, long long u) {
+  int s = 0;
+  while (s + 1 < nseg && u >= segs[s + 1].unit0) ++s;
+  return s;
+}
+
+// Persistent X-stage over several subbands of the same M: unit = (subband,
+// split, channel group), units strided over the grid.
+template <int ET, int MB>
+__global__ void __launch_bounds__(256) xmfma_multi_kernel(const XSeg *segs, int nseg, long long total) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+    const XSeg &g = segs[find_seg(segs, nseg, u)];
+    const int local = (int)(u - g.unit0);
+    const int groups = g.p.nchan / XM<ET, MB>::CT;
+    __syncthreads();  // the previous unit's LDS reads are done
+    xmfma_item<ET, MB>(g.p, local % groups, local / groups, lds);
+  }
+}
+
 __device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
 #pragma clang fp contract(off)
   const float t = sum + v;
@@ -224,12 +253,10 @@ __device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
   sum = t;
 }
 
-__global__ void __launch_bounds__(256) finalise_mixed_kernel(FinMParams p) {
-  // Compensated sum of the split partials, Im from the antisymmetric part of
-  // G_IR, undo the per-(input, channel) scales, normalise by the row count
-  // (fxkernel.cpp:358-367).
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int prod = blockIdx.y, row = blockIdx.z;
+// Compensated sum of the split partials, Im from the antisymmetric part of
+// G_IR, undo the per-(input, channel) scales, normalise by the row count
+// (fxkernel.cpp:358-367): channel c of product prod of row `row`.
+__device__ __forceinline__ void finalise_one(const FinMParams &p, int c, int prod, int row) {
   if (c >= p.nchan) return;
   const int2 jk = p.rowmap[row];
   const int a = 2 * jk.x + (prod >> 1), b = 2 * jk.y + (prod & 1);
@@ -256,21 +283,47 @@ __global__ void __launch_bounds__(256) finalise_mixed_kernel(FinMParams p) {
   p.vis[((size_t)row * 4 + prod) * p.nchan + c] = make_float2(re, im);
 }
 
+__global__ void __launch_bounds__(256) finalise_mixed_kernel(FinMParams p) {
+  finalise_one(p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// unit = (row, product, 256-channel chunk) of one subband
+__global__ void __launch_bounds__(256) finalise_multi_kernel(const FinSeg *segs, int nseg, long long total) {
+  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+    const FinSeg &g = segs[find_seg(segs, nseg, u)];
+    const int local = (int)(u - g.unit0);
+    const int nck = (g.p.nchan + 255) / 256;
+    finalise_one(g.p, (local % nck) * 256 + threadIdx.x, (local / nck) % 4, local / (4 * nck));
+  }
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void scales_kernel(float *scales, const float *power, int n, int nrows, float target) {
-  // Mean |X|^2 over the sampled FFTs (rows summed in order), then the power
-  // of two nearest target/rms: scaling and unscaling are exact, and the
-  // scale is insensitive to the last bits of the estimate.
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Mean |X|^2 over the sampled FFTs (rows summed in order), then the power of
+// two nearest target/rms: scaling and unscaling are exact, and the scale is
+// insensitive to the last bits of the estimate.
+__device__ __forceinline__ void scale_one(float *scales, const float *power, int n, int nrows, float target,
+                                          int i) {
   if (i >= n) return;
   float sum = 0.f;
   for (int r = 0; r < nrows; r++) sum += power[(size_t)r * n + i];
   const float pw = sum / (float)nrows;
   scales[i] = pw > 0.f ? exp2f(rintf(log2f(target / sqrtf(pw)))) : 1.0f;
+}
+
+__global__ void scales_kernel(float *scales, const float *power, int n, int nrows, float target) {
+  scale_one(scales, power, n, nrows, target, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// unit = 256 consecutive (input, channel) entries of one subband
+__global__ void __launch_bounds__(256) scales_multi_kernel(const ScaleSeg *segs, int nseg, long long total) {
+  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+    const ScaleSeg &g = segs[find_seg(segs, nseg, u)];
+    scale_one(g.scales, g.power, g.n, g.nrows, g.target, (int)(u - g.unit0) * 256 + threadIdx.x);
+  }
 }
 
 __device__ __forceinline__ uint32_t q_bf16_1(float x) {
@@ -310,6 +363,32 @@ hipError_t launch_xm(const XMParams &p, hipStream_t s) {
   }
   hipLaunchKernelGGL((xmfma_kernel<ET, MB>), grid, dim3(256), X::LDSB, s, p);
   return hipGetLastError();
+}
+
+template <int ET, int MB>
+hipError_t launch_xm_multi(const XSeg *segs, int nseg, long long total, hipStream_t s) {
+  using X = XM<ET, MB>;
+  const void *fn = reinterpret_cast<const void *>(&xmfma_multi_kernel<ET, MB>);
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  const long long grid = persistent_grid(fn, 256, X::LDSB, total);
+  hipLaunchKernelGGL((xmfma_multi_kernel<ET, MB>), dim3((unsigned)grid), dim3(256), X::LDSB, s, segs, nseg,
+                     total);
+  return hipGetLastError();
+}
+
+template <int ET>
+hipError_t launch_xm_multi_mb(const XSeg *segs, int nseg, long long total, int M, hipStream_t s) {
+  switch (xm_blocks(M)) {
+    case 1: return launch_xm_multi<ET, 1>(segs, nseg, total, s);
+    case 2: return launch_xm_multi<ET, 2>(segs, nseg, total, s);
+    case 4: return launch_xm_multi<ET, 4>(segs, nseg, total, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <int ET>
@@ -370,6 +449,33 @@ hipError_t launch_xmfma(const XMParams &p, int out_mode, hipStream_t s) {
     case OUT_FP8: return launch_xm_mb<OUT_FP8>(p, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_xmfma_multi(const XSeg *segs, int nseg, long long total, int out_mode, int M, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  switch (out_mode) {
+    case OUT_F32: return launch_xm_multi_mb<OUT_F32>(segs, nseg, total, M, s);
+    case OUT_F16: return launch_xm_multi_mb<OUT_F16>(segs, nseg, total, M, s);
+    case OUT_BF16: return launch_xm_multi_mb<OUT_BF16>(segs, nseg, total, M, s);
+    case OUT_FP8: return launch_xm_multi_mb<OUT_FP8>(segs, nseg, total, M, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const void *fn = reinterpret_cast<const void *>(&finalise_multi_kernel);
+  const long long grid = persistent_grid(fn, 256, 0, total);
+  hipLaunchKernelGGL(finalise_multi_kernel, dim3((unsigned)grid), dim3(256), 0, s, segs, nseg, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_scales_multi(const ScaleSeg *segs, int nseg, long long total, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const void *fn = reinterpret_cast<const void *>(&scales_multi_kernel);
+  const long long grid = persistent_grid(fn, 256, 0, total);
+  hipLaunchKernelGGL(scales_multi_kernel, dim3((unsigned)grid), dim3(256), 0, s, segs, nseg, total);
+  return hipGetLastError();
 }
 
 hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s) {

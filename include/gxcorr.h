@@ -188,6 +188,67 @@ gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft,
 gxc_status gxc_set_profiling(gxc_handle *h, int enable);
 gxc_status gxc_get_timings(const gxc_handle *h, gxc_timings *t);
 
+/* ---- Several subbands per launch ------------------------------------------
+ * fxkernel correlates one band per FxKernel object (fxkernel/src/fxkernel.h:29);
+ * an observation with several subbands runs one object per subband.  A batch
+ * holds every subband of a subintegration: the same stations, each subband
+ * with its own channel count, bandwidth, LO, FFT count, delays and input.
+ * Each stage runs for all subbands in one persistent launch (F-stage,
+ * X-stage, row counts, finalise), and with graph on the whole subintegration
+ * is captured as one hipGraph.  Per subband the visibilities are those of a
+ * gxc_handle with the same settings (same layout, same normalisation).
+ * Real 2-bit input, FFT lengths 2*nchan of 256..8192. */
+typedef struct {
+  int nchan;    /* NCHAN of this subband, power of two, 128..4096 */
+  int numffts;  /* NUMFFTS of this subband's subintegration */
+  double lo;    /* LO, Hz */
+  double bw;    /* BANDWIDTH, Hz */
+} gxc_subband;
+
+typedef struct {
+  int nant;                     /* stations, shared by all subbands */
+  int nbit;                     /* 2 */
+  int npol;                     /* 2 */
+  int is_complex;               /* 0 */
+  int precision;                /* gxc_precision */
+  int autos;                    /* 1: append autocorrelation rows */
+  int exact_phase;              /* as gxc_config.exact_phase */
+  int nsub;                     /* number of subbands */
+  const gxc_subband *subbands;  /* [nsub], copied by gxc_batch_create */
+  int chunks;                   /* device passes per subintegration (each
+                                   covers 1/chunks of every subband's FFTs);
+                                   0 = automatic */
+  int overlap;                  /* as gxc_config.overlap */
+  int graph;                    /* as gxc_config.graph */
+  int reserved[4];
+} gxc_batch_config;
+
+typedef struct gxc_batch gxc_batch;
+
+/* Defaults: nbit 2, npol 2, real, FP32, no autos, automatic chunks, overlap
+ * and graph on, no subbands. */
+void gxc_batch_config_init(gxc_batch_config *cfg);
+gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out);
+void gxc_batch_destroy(gxc_batch *b);
+const char *gxc_batch_last_error(const gxc_batch *b);
+/* Per subband `sub`, as gxc_set_input / gxc_set_input_device / gxc_set_delays
+ * (station buffers of gxc_station_bytes() of that subband's configuration). */
+gxc_status gxc_batch_set_input(gxc_batch *b, int sub, const uint8_t *const *per_station);
+gxc_status gxc_batch_set_input_device(gxc_batch *b, int sub, const void *const *per_station);
+gxc_status gxc_batch_set_delays(gxc_batch *b, int sub, const double *const *poly3,
+                                const double *fileoffsets);
+/* One subintegration of every subband. */
+gxc_status gxc_batch_process(gxc_batch *b);
+gxc_status gxc_batch_process_async(gxc_batch *b);
+gxc_status gxc_batch_synchronize(gxc_batch *b);
+/* Subband `sub`'s visibilities [rows][4][nchan] and row counts. */
+gxc_status gxc_batch_get_visibilities(gxc_batch *b, int sub, gxc_cf32 *out, size_t nelem);
+gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n);
+/* Timing over all subbands (launch counts are per-stage launches, one per
+ * chunk for F and X). */
+gxc_status gxc_batch_set_profiling(gxc_batch *b, int enable);
+gxc_status gxc_batch_get_timings(const gxc_batch *b, gxc_timings *t);
+
 /* .conf reader, fxkernel/src/common.cpp:190-250 (same keywords, same
  * defaults, unknown keywords reported and skipped).  The strings and arrays
  * are owned by the struct and released by gxc_conffile_free. */

@@ -137,3 +137,29 @@ def test_product_does_not_reference_oracle():
             assert not re.search(r"^\s*(from|import)\s+oracle", fh.read(), re.M), f
     with open(_lib.LIB_PATH, "rb") as fh:
         assert b"fxref" not in fh.read()
+
+
+def test_batch_create_checks_config_before_device():
+    L = _lib.load()
+
+    def create(subs, **kw):
+        c = _lib.BatchConfig()
+        L.gxc_batch_config_init(ctypes.byref(c))
+        c.nant = 8
+        arr = (_lib.Subband * max(1, len(subs)))(*[_lib.Subband(*s) for s in subs])
+        c.nsub = len(subs)
+        c.subbands = arr
+        for k, v in kw.items():
+            setattr(c, k, v)
+        h = ctypes.c_void_p()
+        return L.gxc_batch_create(ctypes.byref(c), ctypes.byref(h))
+
+    ok = [(256, 100, 1.65e9, 16e6), (4096, 50, 1.65e9, 128e6)]
+    assert create([]) == 1                                     # no subbands
+    assert create(ok, nbit=8, is_complex=1) == 2               # real 2-bit only
+    assert create([(64, 100, 1.65e9, 16e6)]) == 2              # FFT length 128 < 256
+    assert create([(8192, 10, 1.65e9, 16e6)]) == 2             # FFT length 16384 > 8192
+    assert create([(100, 10, 1.65e9, 16e6)]) == 1              # not a power of two
+    assert create(ok) == 3                                     # valid, but no GPU here
+    msg = L.gxc_batch_last_error(None).decode()
+    assert "device" in msg
