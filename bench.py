@@ -40,7 +40,8 @@ def parse():
                     help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-h2d", action="store_true")
-    ap.add_argument("--no-overlap", action="store_true", help="F and X on one stream")
+    ap.add_argument("--overlap", action="store_true",
+                    help="F of chunk k+1 on a second stream beside X of chunk k")
     ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scales HBM counters)")
@@ -125,14 +126,14 @@ def main():
         del src, dst
     if args.workload == "c4":
         k = gcorr_amd.Batch(w.nant, [(x.nchan, x.numffts, x.lo, x.bw) for x in ws],
-                            precision=args.precision, autos=True, overlap=not args.no_overlap)
+                            precision=args.precision, autos=True, overlap=args.overlap)
         for sidx, (bufs, poly, offs) in enumerate(gens):
             k.set_input_device(sidx, [b.data_ptr() for b in bufs], keepalive=bufs)
             k.set_delays(sidx, poly, offs)
     else:
         bufs, poly, offs = gens[0]
         k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
-                               precision=args.precision, autos=True, overlap=not args.no_overlap,
+                               precision=args.precision, autos=True, overlap=args.overlap,
                                chunk_ffts=args.chunk_ffts)
         k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
         k.setDelays(poly, offs)
@@ -246,7 +247,7 @@ def main():
             "bandwidth_hz": [x.bw for x in ws] if len(ws) > 1 else w.bw, "lo_hz": w.lo,
             "input": "complex int8" if w.is_complex else "real 2-bit",
             "precision": args.precision, "autos": True,
-            "overlap": not args.no_overlap, "chunk_ffts": args.chunk_ffts or "auto",
+            "overlap": args.overlap, "chunk_ffts": args.chunk_ffts or "auto",
             "input_samples_per_step": input_samples,
         },
         "roofline": roofline,

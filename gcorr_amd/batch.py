@@ -19,7 +19,7 @@ class Batch:
     """Batch(nant, [(nchan, numffts, lo, bw), ...])."""
 
     def __init__(self, nant, subbands, *, precision="fp32", autos=False, exact_phase=False,
-                 chunks=0, overlap=True, graph=True):
+                 chunks=0, overlap=False, graph=True):
         self._L = _lib.load()
         c = _lib.BatchConfig()
         self._L.gxc_batch_config_init(ctypes.byref(c))

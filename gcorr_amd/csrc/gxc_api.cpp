@@ -353,7 +353,7 @@ void gxc_config_init(gxc_config *cfg) {
   cfg->npol = 2;
   cfg->is_complex = 0;
   cfg->precision = GXC_PREC_FP32;
-  cfg->overlap = 1;
+  cfg->overlap = 0;  // both stages fill the device: overlap measured no faster
   cfg->graph = 1;
 }
 

@@ -229,7 +229,7 @@ gxc_status build_tables(gxc_batch *b) {
     f.nrows = b->nrows;
     f.splits = u.splits;
     finseg[s].unit0 = fin;
-    fin += (long long)b->nrows * 4 * ((u.cfg.nchan + 255) / 256);
+    fin += u.cfg.nchan;
   }
   b->counttotal = cu;
   b->fintotal = fin;
@@ -291,7 +291,7 @@ gxc_status enqueue_device_work(gxc_batch *b) {
   }
   mark(b, 4, xs);
   GXB_HIP(b, gxc::launch_counts_multi(b->d_countseg, ns, b->counttotal, xs));
-  GXB_HIP(b, gxc::launch_finalise_multi(b->d_finseg, ns, b->fintotal, xs));
+  GXB_HIP(b, gxc::launch_finalise_multi(b->d_finseg, ns, b->fintotal, b->M, xs));
   mark(b, 5, xs);
   if (overlap) {
     GXB_HIP(b, hipEventRecord(b->ev_join, xs));
@@ -395,7 +395,7 @@ void gxc_batch_config_init(gxc_batch_config *cfg) {
   cfg->nbit = 2;
   cfg->npol = 2;
   cfg->precision = GXC_PREC_FP32;
-  cfg->overlap = 1;
+  cfg->overlap = 0;
   cfg->graph = 1;
 }
 
@@ -467,9 +467,16 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
     if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
       return fail(set_err(b, GXC_ERR_HIP, "hipEventCreate failed"));
 
-  // chunks: automatic = 8 passes with overlap, else one; at most the
-  // smallest subband's FFT count, so every chunk has FFTs of every subband
-  int C = cfg->chunks > 0 ? cfg->chunks : (cfg->overlap ? 8 : 1);
+  // chunks: automatic = 8 passes with overlap, else as few as keep one
+  // spectra buffer within 8 GiB; at most the smallest subband's FFT count,
+  // so every chunk has FFTs of every subband
+  int C = cfg->chunks;
+  if (C <= 0) {
+    double total = 0;
+    for (const Sub &u : b->sub)
+      total += (double)u.cfg.numffts * 2 * cfg->nant * u.cfg.nchan * gxc::xm_element_bytes(out_mode);
+    C = cfg->overlap ? 8 : (int)std::ceil(total / (8.0 * (1ull << 30)));
+  }
   C = std::max(1, std::min(C, minfft));
   b->nchunks = C;
   b->nbuf = (cfg->overlap && C > 1) ? 2 : 1;

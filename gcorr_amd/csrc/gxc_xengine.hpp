@@ -33,7 +33,7 @@ struct XSeg {  // units = (nchan / channel tile) * splits
   XMParams p;
   long long unit0;
 };
-struct FinSeg {  // units = nrows * 4 * ceil(nchan / 256)
+struct FinSeg {  // units = nchan
   FinMParams p;
   long long unit0;
 };
@@ -64,7 +64,7 @@ hipError_t launch_quantise(void *out, const float2 *in, size_t n, int out_mode, 
 // Persistent launches over device-resident segment tables (all subbands of
 // one X launch share M).
 hipError_t launch_xmfma_multi(const XSeg *segs, int nseg, long long total, int out_mode, int M, hipStream_t s);
-hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, hipStream_t s);
+hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, int M, hipStream_t s);
 hipError_t launch_scales_multi(const ScaleSeg *segs, int nseg, long long total, hipStream_t s);
 // RMS that the per-(input, channel) scales normalise the spectra to.
 float xm_target_rms(int out_mode);

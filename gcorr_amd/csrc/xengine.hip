@@ -14,7 +14,15 @@
 // [TB FFTs][M][CT channels] tile through LDS, transposed per channel so
 // that a lane's MFMA operand is contiguous there: [plane][input][t] for the
 // 8-element operands of the 16/8-bit types, [plane][t][input] for FP32.
-// Rows of inputs past M stay zero in LDS.
+// The transpose happens in registers: a lane loads 8 FFTs (16/8-bit) or 4
+// inputs (FP32) of one 16-byte column, regroups them with v_perm, and
+// stores whole operands (16, 8 or 16 bytes) to LDS.  Chunk positions are
+// XOR-swizzled (16/8-bit) or rows padded (FP32) so that the operand reads
+// of 16 lanes fall in distinct banks.  Rows of inputs past M stay zero.
+//
+// Split partials are [split][channel][plane][a][b]: a wave's stores (and
+// the read-modify-write of later chunks) cover rows of one channel, and the
+// finalise reads a channel's partials as one contiguous block.
 #include <type_traits>
 
 #include "gxc_xengine.hpp"
@@ -35,17 +43,27 @@ struct XM {
   static constexpr int TB = F32 ? 16 : 32;                   // FFTs per LDS stage
   static constexpr int MP = 16 * MB;                         // inputs padded to blocks
   static constexpr int CT = (F32 ? 16 : 32) / MB;            // channels per workgroup
-  static constexpr int PLANEB = MP * TB * EB;
+  static constexpr int NTILE = MB * (MB + 1) / 2;
+  static constexpr int EPL = 16 / CB;  // complex values (channels) per 16-byte load
+  static constexpr int CPW = CT / 4;   // channels per wave
+  // 16/8-bit: per plane, rows m of TB*EB bytes = TB/8 chunks of 8 FFTs.
+  // FP32: per plane, rows t of ROWB bytes holding the inputs contiguously,
+  // padded so that 4 consecutive rows start 64 or 192 bytes apart mod 256.
+  static constexpr int CHUNKB = 8 * EB;
+  static constexpr int ROWB = F32 ? MP * 4 + (MB == 1 ? 0 : 64) : TB * EB;
+  static constexpr int PLANEB = F32 ? TB * ROWB : MP * ROWB;
   static constexpr int CHB = 2 * PLANEB + 16;  // +16 B spreads channels over banks
   static constexpr int LDSB = CT * CHB;
-  static constexpr int NTILE = MB * (MB + 1) / 2;
-  static constexpr int EPL = 16 / CB;   // complex values per 16-byte load
-  static constexpr int TPR = CT / EPL;  // 16-byte loads per (t, input) row
-  static constexpr int CPW = CT / 4;    // channels per wave
-  // byte offset of component (plane, input m, FFT t) within a channel
+  // chunk swizzle of the 16/8-bit layout: the 16 inputs of one MFMA block
+  // read their chunk t/8 from 16 distinct bank groups
+  __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : (m >> 3) & 3; }
+  // byte offset of component (plane, input m, FFT t) within a channel;
+  // 16/8-bit: t a multiple of 8 (the start of an operand)
   __device__ static constexpr int at(int plane, int m, int t) {
-    return plane * PLANEB + (F32 ? (t * MP + m) : (m * TB + t)) * EB;
+    return F32 ? plane * PLANEB + t * ROWB + m * 4
+               : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m)) & (TB / 8 - 1)) * CHUNKB;
   }
+  static_assert(F32 || TB / 8 == 4, "4 chunks per row");
 };
 
 __device__ __forceinline__ floatx4 mfma(const uint4 &a, const uint4 &b, floatx4 c,
@@ -74,26 +92,96 @@ struct Operand {
       ET == OUT_F32, float, typename std::conditional<ET == OUT_FP8, uint2, uint4>::type>::type;
 };
 
-// Scatter one 16-byte load (EPL complex values of consecutive channels at
-// FFT t, input m) into the per-channel transposed LDS layout.
+// Load one stage's [TB FFTs][M][CT] tile (FFTs ts.. of rows < te) and store
+// it transposed into LDS.  Tasks are strided over the 256 threads; in a
+// task the channel column is the fastest index, so neighbouring lanes load
+// neighbouring 16-byte columns of one spectra row.
 template <int ET, int MB>
-__device__ __forceinline__ void stage_store(unsigned char *lds, const uint4 &v, int ch0, int m, int t) {
+__device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned char *src, size_t rowbytes,
+                                           int M, int c0, int ts, int te, int tid) {
   using X = XM<ET, MB>;
-  const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-  unsigned char *dst = lds + ch0 * X::CHB;
+  constexpr int NQ = X::CT / X::EPL;  // 16-byte columns per row
+  if constexpr (X::F32) {
+    // task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads in flight,
+    // then per channel, plane and 4 inputs one 16-byte store
+    const int nmq = (M + 7) / 8;
+    const int ntask = NQ * nmq * X::TB;
+    for (int task = tid; task < ntask; task += 256) {
+      const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
+      uint4 w[8];
 #pragma unroll
-  for (int e = 0; e < X::EPL; e++) {
-    unsigned char *d = dst + e * X::CHB;
-    if constexpr (X::EB == 4) {
-      *reinterpret_cast<uint32_t *>(d + X::at(0, m, t)) = w4[2 * e];
-      *reinterpret_cast<uint32_t *>(d + X::at(1, m, t)) = w4[2 * e + 1];
-    } else if constexpr (X::EB == 2) {
-      *reinterpret_cast<uint16_t *>(d + X::at(0, m, t)) = (uint16_t)(w4[e] & 0xffff);
-      *reinterpret_cast<uint16_t *>(d + X::at(1, m, t)) = (uint16_t)(w4[e] >> 16);
-    } else {
-      const uint32_t word = w4[e >> 1] >> (16 * (e & 1));
-      d[X::at(0, m, t)] = (unsigned char)(word & 0xff);
-      d[X::at(1, m, t)] = (unsigned char)((word >> 8) & 0xff);
+      for (int k = 0; k < 8; k++) {
+        const int m = 8 * mq + k;
+        w[k] = make_uint4(0, 0, 0, 0);
+        if (m < M && ts + t < te)
+          w[k] = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
+                                                  (size_t)(c0 + q * 2) * X::CB);
+      }
+      unsigned char *d0 = lds + (q * 2) * X::CHB, *d1 = d0 + X::CHB;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint4 *v = w + 4 * h;
+        if (8 * mq + 4 * h >= X::MP) break;  // rows past the padded block count
+        const int m0 = 8 * mq + 4 * h;
+        *reinterpret_cast<uint4 *>(d0 + X::at(0, m0, t)) = make_uint4(v[0].x, v[1].x, v[2].x, v[3].x);
+        *reinterpret_cast<uint4 *>(d0 + X::at(1, m0, t)) = make_uint4(v[0].y, v[1].y, v[2].y, v[3].y);
+        *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
+        *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
+      }
+    }
+  } else {
+    // task = (column q, input m, FFTs 8*o..8*o+7): 8 loads, then per
+    // channel and plane one store of the 8 FFTs' components
+    const int ntask = NQ * M * (X::TB / 8);
+    for (int task = tid; task < ntask; task += 256) {
+      const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
+      uint32_t w[8][4];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int t = 8 * o + k;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (ts + t < te)
+          v = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
+                                               (size_t)(c0 + q * X::EPL) * X::CB);
+        w[k][0] = v.x;
+        w[k][1] = v.y;
+        w[k][2] = v.z;
+        w[k][3] = v.w;
+      }
+      const int off0 = X::at(0, m, 8 * o), off1 = X::at(1, m, 8 * o);
+      if constexpr (X::EB == 2) {
+        // word e of a load = (re | im << 16) of channel q*4+e
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          uint32_t re[4], im[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            re[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x05040100u);
+            im[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x07060302u);
+          }
+          unsigned char *d = lds + (q * 4 + e) * X::CHB;
+          *reinterpret_cast<uint4 *>(d + off0) = make_uint4(re[0], re[1], re[2], re[3]);
+          *reinterpret_cast<uint4 *>(d + off1) = make_uint4(im[0], im[1], im[2], im[3]);
+        }
+      } else {
+        // channel e = bytes 2(e&1) (re), 2(e&1)+1 (im) of word e>>1
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int wi = e >> 1, bb = 2 * (e & 1);
+          const uint32_t sel = (uint32_t)bb | (uint32_t)(4 + bb) << 8 | (uint32_t)(bb + 1) << 16 |
+                               (uint32_t)(5 + bb) << 24;
+          uint32_t x[4];  // [re_2j, re_2j+1, im_2j, im_2j+1]
+#pragma unroll
+          for (int j = 0; j < 4; j++) x[j] = __builtin_amdgcn_perm(w[2 * j + 1][wi], w[2 * j][wi], sel);
+          const uint32_t re0 = __builtin_amdgcn_perm(x[1], x[0], 0x05040100u);
+          const uint32_t re1 = __builtin_amdgcn_perm(x[3], x[2], 0x05040100u);
+          const uint32_t im0 = __builtin_amdgcn_perm(x[1], x[0], 0x07060302u);
+          const uint32_t im1 = __builtin_amdgcn_perm(x[3], x[2], 0x07060302u);
+          unsigned char *d = lds + (q * 8 + e) * X::CHB;
+          *reinterpret_cast<uint2 *>(d + off0) = make_uint2(re0, re1);
+          *reinterpret_cast<uint2 *>(d + off1) = make_uint2(im0, im1);
+        }
+      }
     }
   }
 }
@@ -105,7 +193,6 @@ template <int ET, int MB>
 __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int split, unsigned char *lds) {
   using X = XM<ET, MB>;
   using Op = typename Operand<ET>::type;
-  constexpr int NBATCH = 8;  // 16-byte loads issued before their LDS stores
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c0 = group * X::CT;
   const int tb = (int)((long long)p.nt * split / p.splits);
@@ -122,33 +209,9 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
 
   const unsigned char *src = static_cast<const unsigned char *>(p.spectra);
   const size_t rowbytes = (size_t)p.nchan * X::CB;
-  const int nload = X::TB * M * X::TPR;  // 16-byte loads per stage
   for (int ts = tb; ts < te; ts += X::TB) {
     __syncthreads();  // the previous stage's operand reads are done
-    for (int i0 = tid; i0 < nload; i0 += 256 * NBATCH) {
-      uint4 v[NBATCH];
-#pragma unroll
-      for (int u = 0; u < NBATCH; u++) {
-        const int idx = i0 + u * 256;
-        v[u] = make_uint4(0, 0, 0, 0);
-        if (idx < nload) {
-          const int piece = idx % X::TPR, row = idx / X::TPR;
-          const int t = row / M, m = row - t * M;
-          if (ts + t < te)
-            v[u] = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
-                                                    (size_t)(c0 + piece * X::EPL) * X::CB);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < NBATCH; u++) {
-        const int idx = i0 + u * 256;
-        if (idx < nload) {
-          const int piece = idx % X::TPR, row = idx / X::TPR;
-          const int t = row / M, m = row - t * M;
-          stage_store<ET, MB>(lds, v[u], piece * X::EPL, m, t);
-        }
-      }
-    }
+    stage_tile<ET, MB>(lds, src, rowbytes, M, c0, ts, te, tid);
     __syncthreads();
     // Wave w: channels w, w+4, ...  Lane l's operand: input l%16 of a block
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
@@ -181,6 +244,8 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   }
 
   // D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4.
+  // Partials: [split][channel][plane][a][b], so a wave's stores and the
+  // read-modify-write of later chunks cover whole rows of one channel.
   const size_t MM = (size_t)M * M;
 #pragma unroll
   for (int w = 0; w < X::CPW; w++) {
@@ -198,16 +263,17 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
           const int b = bj * 16 + (lane & 15);
           if (a < M && b < M) {
             float vre = acc_re[w][tile][r], vir = acc_ir[w][tile][r];
+            const size_t ab = (size_t)a * M + b, ba = (size_t)b * M + a;
             if (p.accumulate) {
-              vre += pre[(size_t)a * M + b];
-              vir += pir[(size_t)a * M + b];
+              vre += pre[ab];
+              vir += pir[ab];
             }
-            pre[(size_t)a * M + b] = vre;
-            pir[(size_t)a * M + b] = vir;
+            pre[ab] = vre;
+            pir[ab] = vir;
             if (bi != bj) {  // R_bi I_bj^T holds G_IR transposed
               float vri = acc_ri[w][tile][r];
-              if (p.accumulate) vri += pir[(size_t)b * M + a];
-              pir[(size_t)b * M + a] = vri;
+              if (p.accumulate) vri += pir[ba];
+              pir[ba] = vri;
             }
           }
         }
@@ -215,10 +281,24 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   }
 }
 
+// Workgroups are dealt to the 8 XCDs round robin in dispatch order, and each
+// XCD has its own L2.  A channel group's rows are CT*CB bytes wide, less
+// than a 128-byte line when CT is small, so neighbouring groups share lines:
+// renumber the units so that consecutive units (neighbouring channel groups)
+// run on the same XCD one after another and meet the line in its L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ long long xcd_swizzle(long long u, long long total) {
+  const long long per = total / kXcds;  // units per XCD in the swizzled range
+  if (u >= per * kXcds) return u;       // tail: unchanged
+  return (u % kXcds) * per + u / kXcds;
+}
+
 template <int ET, int MB>
 __global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  xmfma_item<ET, MB>(p, blockIdx.x, blockIdx.y, lds);
+  const int groups = p.nchan / XM<ET, MB>::CT;
+  const long long u = xcd_swizzle(blockIdx.x, (long long)groups * p.splits);
+  xmfma_item<ET, MB>(p, (int)(u % groups), (int)(u / groups), lds);
 }
 
 // Segment of a multi-subband launch holding unit u (segments in order).
@@ -234,7 +314,10 @@ __device__ __forceinline__ int find_seg(const Seg *segs, int nseg, long long u) 
 template <int ET, int MB>
 __global__ void __launch_bounds__(256) xmfma_multi_kernel(const XSeg *segs, int nseg, long long total) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+  for (long long v = blockIdx.x; v < total; v += gridDim.x) {
+    // gridDim.x is a multiple of the XCD count, so v % kXcds is this
+    // workgroup's XCD and the swizzle keeps neighbouring units on it
+    const long long u = xcd_swizzle(v, total);
     const XSeg &g = segs[find_seg(segs, nseg, u)];
     const int local = (int)(u - g.unit0);
     const int groups = g.p.nchan / XM<ET, MB>::CT;
@@ -253,47 +336,59 @@ __device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
   sum = t;
 }
 
-// Compensated sum of the split partials, Im from the antisymmetric part of
-// G_IR, undo the per-(input, channel) scales, normalise by the row count
-// (fxkernel.cpp:358-367): channel c of product prod of row `row`.
-__device__ __forceinline__ void finalise_one(const FinMParams &p, int c, int prod, int row) {
-  if (c >= p.nchan) return;
-  const int2 jk = p.rowmap[row];
-  const int a = 2 * jk.x + (prod >> 1), b = 2 * jk.y + (prod & 1);
-  const int lo = a < b ? a : b, hi = a < b ? b : a;
-  const size_t MM = (size_t)p.M * p.M;
-  float sre = 0.f, cre = 0.f, sim = 0.f, cim = 0.f;
-  for (int s = 0; s < p.splits; s++) {
-    const float *pre = p.partial + ((size_t)s * p.nchan + c) * 2 * MM;
-    const float *pir = pre + MM;
-    neumaier(sre, cre, pre[(size_t)lo * p.M + hi]);
-    neumaier(sim, cim, pir[(size_t)a * p.M + b] - pir[(size_t)b * p.M + a]);
+// Finalise of channel c on a workgroup of 256 threads: the split partials
+// of the channel (2 M^2 floats each, contiguous) are summed with Neumaier
+// compensation in split order into LDS, then every (row, product) of the
+// channel is formed there: Re from the upper block triangle, Im from the
+// antisymmetric part of G_IR, the per-(input, channel) scales divided out,
+// normalised by the row count (fxkernel.cpp:358-367).  sh holds 2 M^2 floats.
+__device__ __forceinline__ void finalise_channel(const FinMParams &p, int c, float *sh) {
+  const int M = p.M;
+  const size_t MM = (size_t)M * M;
+  const int tid = threadIdx.x;
+  __syncthreads();  // the previous channel's reads of sh are done
+  for (int e = tid; e < 2 * (int)MM; e += 256) {
+    float sum = 0.f, comp = 0.f;
+    for (int s = 0; s < p.splits; s++) neumaier(sum, comp, p.partial[((size_t)s * p.nchan + c) * 2 * MM + e]);
+    sh[e] = sum + comp;
   }
-  float re = sre + cre, im = sim + cim;
-  if (p.scales) {
-    const float sc = p.scales[(size_t)a * p.nchan + c] * p.scales[(size_t)b * p.nchan + c];
-    re /= sc;
-    im /= sc;
+  __syncthreads();
+  const float *re_pl = sh, *ir_pl = sh + MM;
+  for (int o = tid; o < 4 * p.nrows; o += 256) {
+    const int row = o >> 2, prod = o & 3;
+    const int2 jk = p.rowmap[row];
+    const int a = 2 * jk.x + (prod >> 1), b = 2 * jk.y + (prod & 1);
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    float re = re_pl[(size_t)lo * M + hi];
+    float im = ir_pl[(size_t)a * M + b] - ir_pl[(size_t)b * M + a];
+    if (p.scales) {
+      const float sc = p.scales[(size_t)a * p.nchan + c] * p.scales[(size_t)b * p.nchan + c];
+      re /= sc;
+      im /= sc;
+    }
+    const int cnt = p.counts[row];
+    if (cnt != 0) {
+      re /= (float)cnt;
+      im /= (float)cnt;
+    }
+    p.vis[((size_t)row * 4 + prod) * p.nchan + c] = make_float2(re, im);
   }
-  const int cnt = p.counts[row];
-  if (cnt != 0) {
-    re /= (float)cnt;
-    im /= (float)cnt;
-  }
-  p.vis[((size_t)row * 4 + prod) * p.nchan + c] = make_float2(re, im);
 }
 
+// One workgroup per channel; channels swizzled so that neighbours run on
+// one XCD and their visibility stores meet in its L2.
 __global__ void __launch_bounds__(256) finalise_mixed_kernel(FinMParams p) {
-  finalise_one(p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+  extern __shared__ float sh[];
+  finalise_channel(p, (int)xcd_swizzle(blockIdx.x, p.nchan), sh);
 }
 
-// unit = (row, product, 256-channel chunk) of one subband
+// unit = (subband, channel)
 __global__ void __launch_bounds__(256) finalise_multi_kernel(const FinSeg *segs, int nseg, long long total) {
-  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+  extern __shared__ float sh[];
+  for (long long v = blockIdx.x; v < total; v += gridDim.x) {
+    const long long u = xcd_swizzle(v, total);
     const FinSeg &g = segs[find_seg(segs, nseg, u)];
-    const int local = (int)(u - g.unit0);
-    const int nck = (g.p.nchan + 255) / 256;
-    finalise_one(g.p, (local % nck) * 256 + threadIdx.x, (local / nck) % 4, local / (4 * nck));
+    finalise_channel(g.p, (int)(u - g.unit0), sh);
   }
 }
 
@@ -353,7 +448,9 @@ template <int ET, int MB>
 hipError_t launch_xm(const XMParams &p, hipStream_t s) {
   using X = XM<ET, MB>;
   if (p.nchan % X::CT) return hipErrorInvalidValue;
-  dim3 grid(p.nchan / X::CT, p.splits);
+  const long long units = (long long)(p.nchan / X::CT) * p.splits;
+  if (units > 0x7fffffffLL) return hipErrorInvalidValue;
+  dim3 grid((unsigned)units);
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be allowed once per kernel
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB>),
@@ -462,11 +559,12 @@ hipError_t launch_xmfma_multi(const XSeg *segs, int nseg, long long total, int o
   }
 }
 
-hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, hipStream_t s) {
+hipError_t launch_finalise_multi(const FinSeg *segs, int nseg, long long total, int M, hipStream_t s) {
   if (total <= 0) return hipSuccess;
   const void *fn = reinterpret_cast<const void *>(&finalise_multi_kernel);
-  const long long grid = persistent_grid(fn, 256, 0, total);
-  hipLaunchKernelGGL(finalise_multi_kernel, dim3((unsigned)grid), dim3(256), 0, s, segs, nseg, total);
+  const size_t lds = 2 * (size_t)M * M * sizeof(float);
+  const long long grid = persistent_grid(fn, 256, lds, total);
+  hipLaunchKernelGGL(finalise_multi_kernel, dim3((unsigned)grid), dim3(256), lds, s, segs, nseg, total);
   return hipGetLastError();
 }
 
@@ -491,8 +589,8 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {
 
 hipError_t launch_finalise_mixed(const FinMParams &p, hipStream_t s) {
   if (p.nrows <= 0) return hipSuccess;
-  dim3 grid((p.nchan + 255) / 256, 4, p.nrows);
-  hipLaunchKernelGGL(finalise_mixed_kernel, grid, dim3(256), 0, s, p);
+  const size_t lds = 2 * (size_t)p.M * p.M * sizeof(float);  // at most 32 KiB (M <= 64)
+  hipLaunchKernelGGL(finalise_mixed_kernel, dim3(p.nchan), dim3(256), lds, s, p);
   return hipGetLastError();
 }
 

@@ -16,7 +16,7 @@ from ._lib import check
 
 
 def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact_phase,
-            chunk_ffts, overlap=True, graph=True):
+            chunk_ffts, overlap=False, graph=True):
     c = _lib.Config()
     _lib.load().gxc_config_init(ctypes.byref(c))
     c.nant, c.nchan, c.numffts, c.nbit = int(nant), int(nchan), int(nfft), int(nbit)
@@ -46,7 +46,7 @@ class FxKernel:
     """FxKernel(nant, nchan, nfft, nbit, lo, bw, is_complex) on the MI355X path."""
 
     def __init__(self, nant, nchan, nfft, nbit, lo, bw, is_complex=False, *, precision="fp32",
-                 autos=False, exact_phase=False, chunk_ffts=0, overlap=True, graph=True):
+                 autos=False, exact_phase=False, chunk_ffts=0, overlap=False, graph=True):
         self._L = _lib.load()
         self.cfg = _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos,
                            exact_phase, chunk_ffts, overlap, graph)

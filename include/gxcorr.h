@@ -105,7 +105,7 @@ typedef struct {
 } gxc_timings;
 
 /* Fill cfg with defaults: nbit 2, npol 2, real, FP32, no autos, stride law,
- * automatic chunking, overlap and graph replay on. */
+ * automatic chunking, overlap off, graph replay on. */
 void gxc_config_init(gxc_config *cfg);
 
 /* FxKernel::FxKernel (fxkernel/src/fxkernel.cpp:30-180) */
@@ -226,7 +226,7 @@ typedef struct {
 typedef struct gxc_batch gxc_batch;
 
 /* Defaults: nbit 2, npol 2, real, FP32, no autos, automatic chunks, overlap
- * and graph on, no subbands. */
+ * off, graph on, no subbands. */
 void gxc_batch_config_init(gxc_batch_config *cfg);
 gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out);
 void gxc_batch_destroy(gxc_batch *b);
