@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--overlap", action="store_true",
                     help="F of chunk k+1 on a second stream beside X of chunk k")
     ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (the autotuner runs before warmup)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scales HBM counters)")
     return ap.parse_args()
@@ -138,6 +140,12 @@ def main():
         k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
         k.setDelays(poly, offs)
     input_samples = sum(x.input_samples for x in ws)
+    tuning = None
+    if not args.no_autotune:
+        t = time.time()
+        tuning = k.autotune()
+        tuning["seconds"] = time.time() - t
+        print("autotune: %s" % tuning, file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         k.process()
     k.set_profiling(True)
@@ -252,6 +260,7 @@ def main():
         },
         "roofline": roofline,
         "stages": stages,
+        "tuning": tuning,
         "gbps_input": value * (16 if w.is_complex else 2),
         "pcie_inclusive_gsamples_per_s": h2d,
         "cpu_baseline": cpu,

@@ -31,7 +31,17 @@ class Config(ctypes.Structure):
         ("lo", ctypes.c_double), ("bw", ctypes.c_double),
         ("precision", ctypes.c_int), ("autos", ctypes.c_int), ("exact_phase", ctypes.c_int),
         ("chunk_ffts", ctypes.c_int), ("overlap", ctypes.c_int), ("graph", ctypes.c_int),
-        ("reserved", ctypes.c_int * 4),
+        ("fblock", ctypes.c_int), ("xtile", ctypes.c_int), ("xsplits", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("fblock", ctypes.c_int), ("xtile", ctypes.c_int), ("xsplits", ctypes.c_int),
+        ("graph", ctypes.c_int), ("fstage_ms", ctypes.c_double), ("xstage_ms", ctypes.c_double),
+        ("process_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
+        ("candidates", ctypes.c_int),
     ]
 
 
@@ -116,6 +126,7 @@ def load():
         "gxc_xengine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, vp]),
         "gxc_set_profiling": (ctypes.c_int, [H, ctypes.c_int]),
+        "gxc_autotune": (ctypes.c_int, [H, P(Tuning)]),
         "gxc_get_timings": (ctypes.c_int, [H, P(Timings)]),
         "gxc_parse_config": (ctypes.c_int, [ctypes.c_char_p, P(ConfFile)]),
         "gxc_conffile_free": (None, [P(ConfFile)]),
@@ -132,6 +143,7 @@ def load():
         "gxc_batch_get_visibilities": (ctypes.c_int, [H, ctypes.c_int, vp, ctypes.c_size_t]),
         "gxc_batch_get_row_counts": (ctypes.c_int, [H, ctypes.c_int, ip, ctypes.c_size_t]),
         "gxc_batch_set_profiling": (ctypes.c_int, [H, ctypes.c_int]),
+        "gxc_batch_autotune": (ctypes.c_int, [H, P(Tuning)]),
         "gxc_batch_get_timings": (ctypes.c_int, [H, P(Timings)]),
     }
     for name, (res, args) in sig.items():

@@ -114,6 +114,13 @@ class Batch:
             self._h, int(sub), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), out.size), self._h)
         return out
 
+    def autotune(self):
+        """gxc_batch_autotune: F workgroup size, X unit count (splits) and graph
+        replay timed on this batch's input; the fastest is kept."""
+        t = _lib.Tuning()
+        check_batch(self._L.gxc_batch_autotune(self._h, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
+
     def set_profiling(self, on=True):
         check_batch(self._L.gxc_batch_set_profiling(self._h, int(bool(on))), self._h)
 

@@ -1,207 +1,18 @@
-// F-stage for long FFTs (L = 16384, 32768): one workgroup per (station, FFT, pol).
-//
-// Same per-item work as the LDS-resident F-stage (fstage.hip), reshaped for
-// FFTs whose two polarisations do not fit in one CU's LDS together:
-//   - L/32 threads (512 or 1024), 32 points per thread in registers,
-//     radix-32 Stockham passes (then one radix-16 pass for L = 16384);
-//   - one workgroup per (station, FFT, polarisation); the two workgroups of
-//     one (station, FFT) are adjacent in the grid, so the second reads the
-//     shared input words from L2;
-//   - between passes the points go through LDS one component plane at a
-//     time (real parts, then imaginary parts): L floats plus one pad per 32,
-//     135 KiB at L = 32768.
-#include "fstage_common.hpp"
+// Per-subband launch of the register/plane F-stage (fstage_big.hpp): one
+// workgroup of L/32 threads per (station, FFT, pol).  The only F-stage for
+// L = 16384, 32768; at L = 2048..8192 an alternative to fstage_item that the
+// autotuner weighs.
+#include "fstage_big.hpp"
 
 namespace gxc {
 namespace {
 
 using namespace fs;
 
-template <int L, bool CPLX>
-struct BG {
-  using TG = TabGeom<L, CPLX>;
-  static constexpr int LG = ilog2c(L);
-  static constexpr int PT = 32;     // points per thread
-  static constexpr int T = L / PT;  // threads per item = workgroup size
-  static constexpr int N32 = LG / 5;
-  static constexpr int REM = 1 << (LG % 5);
-  static constexpr int NPASS = N32 + (REM > 1 ? 1 : 0);
-  static constexpr int radix(int p) { return p < N32 ? 32 : REM; }
-  static constexpr int ns(int p) {
-    int n = 1;
-    for (int i = 0; i < p; i++) n *= radix(i);
-    return n;
-  }
-  static constexpr int PADL = L + L / 32;
-  static_assert(L == 16384 || L == 32768, "long-FFT F-stage covers L = 16384, 32768");
-  static_assert(T <= 1024, "one workgroup per FFT");
-};
-
-__device__ __forceinline__ int padb(int i) { return i + (i >> 5); }
-
-// padb(i + k*STRIDE) - padb(i) for STRIDE a multiple of 32 (any i), or for
-// STRIDE == 1 when i is a multiple of 32 and k < 32: a compile-time offset.
-template <int STRIDE>
-__device__ __forceinline__ constexpr int padb_step(int k) {
-  return STRIDE == 1 ? k : k * (STRIDE / 32) * 33;
-}
-
-// Store one component of the outputs of pass P (Stockham order).
-template <int L, bool CPLX, int P, int COMP>
-__device__ __forceinline__ void put_plane(const float2 (&v)[32], float *s, int j) {
-  using G = BG<L, CPLX>;
-  constexpr int R = G::radix(P), NS = G::ns(P), NB = 32 / R, T = G::T;
-#pragma unroll
-  for (int bf = 0; bf < NB; bf++) {
-    const int jj = j + bf * T;
-    const int base = (jj / NS) * NS * R + (jj % NS);
-    static_assert(NS % 32 == 0 || (NS == 1 && R == 32), "compile-time padded offsets");
-    float *d = s + padb(base);
-#pragma unroll
-    for (int k = 0; k < R; k++) d[padb_step<NS>(k)] = COMP ? v[bf * R + k].y : v[bf * R + k].x;
-  }
-}
-
-// Load one component of the inputs of pass P.
-template <int L, bool CPLX, int P, int COMP>
-__device__ __forceinline__ void get_plane(float2 (&v)[32], const float *s, int j) {
-  using G = BG<L, CPLX>;
-  constexpr int R = G::radix(P), NB = 32 / R, T = G::T, STR = L / R;
-#pragma unroll
-  for (int bf = 0; bf < NB; bf++) {
-    const int jj = j + bf * T;
-    static_assert(STR % 32 == 0, "compile-time padded offsets");
-    const float *d = s + padb(jj);
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      const float x = d[padb_step<STR>(k)];
-      if (COMP)
-        v[bf * R + k].y = x;
-      else
-        v[bf * R + k].x = x;
-    }
-  }
-}
-
-// Twiddles and butterflies of pass P >= 1.
-template <int L, bool CPLX, int P>
-__device__ __forceinline__ void butterflies(float2 (&v)[32], int j, const float2 *__restrict__ tw) {
-  using G = BG<L, CPLX>;
-  constexpr int R = G::radix(P), NS = G::ns(P), NB = 32 / R, T = G::T;
-#pragma unroll
-  for (int bf = 0; bf < NB; bf++) {
-    const int jj = j + bf * T;
-    const unsigned m = (unsigned)(jj % NS) * (unsigned)(L / (NS * R));
-#pragma unroll
-    for (int k = 1; k < R; k++) v[bf * R + k] = cmul(v[bf * R + k], tw[m * (unsigned)k]);
-    dft<R>(&v[bf * R]);
-  }
-}
-
-// Passes P .. NPASS-1: exchange through LDS, then butterflies.
-template <int L, bool CPLX, int P>
-__device__ __forceinline__ void later_passes(float2 (&v)[32], float *s, int j, const float2 *__restrict__ tw) {
-  using G = BG<L, CPLX>;
-  if constexpr (P < G::NPASS) {
-    put_plane<L, CPLX, P - 1, 0>(v, s, j);
-    __syncthreads();
-    get_plane<L, CPLX, P, 0>(v, s, j);
-    __syncthreads();
-    put_plane<L, CPLX, P - 1, 1>(v, s, j);
-    __syncthreads();
-    get_plane<L, CPLX, P, 1>(v, s, j);
-    __syncthreads();
-    butterflies<L, CPLX, P>(v, j, tw);
-    later_passes<L, CPLX, P + 1>(v, s, j, tw);
-  }
-}
-
 template <int L, bool CPLX, int OUT>
 __global__ void __launch_bounds__((BG<L, CPLX>::T)) fstage_big_kernel(FParams p) {
-  using G = BG<L, CPLX>;
-  using TG = typename G::TG;
-  constexpr int T = G::T, S = TG::S, S2 = TG::S2, NSTEPS = TG::NSTEPS;
-  static_assert(T % S2 == 0, "pass-0 points of one thread share one sub-rotator entry");
-  __shared__ float splane[G::PADL];
-  __shared__ float2 stab[TG::NTAB];
-
-  const int j = threadIdx.x;
-  const int pol = blockIdx.x & 1;
-  const int item = blockIdx.x >> 1;  // grid = 2 * nant * nt exactly
-  const int tl = item / p.nant;
-  const int st = item % p.nant;
-  const int t = p.t0 + tl * p.tstride;
-  const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;  // output row
-
-  int offset = 0;
-  double frac = 0.0, a = 0.0, b = 0.0;
-  // The same for every thread of the workgroup: the branches below that
-  // depend on it are uniform, so every __syncthreads is reached by all.
-  const bool valid = split_offset<L>(p, st, t, offset, frac, a, b);
-  if (j == 0 && pol == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
-  if (!valid) {
-    for (int q = j; q < TG::NCHAN; q += T)
-      store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, make_float2(0.f, 0.f));
-    return;
-  }
-  fill_tables<L, CPLX>(p, frac, a, b, stab, j, T);
-  __syncthreads();
-  const float2 *sub = stab;
-  const float2 *step = sub + S2;
-  const float2 *fsub = step + NSTEPS;
-  const float2 *fstep = fsub + S;
-  const float2 *__restrict__ tw = p.twiddle;
-
-  // Pass-0 point k of thread j is window sample n = j + k*T: its rotator is
-  // sub[n % S2] * step[n / S2] (fxkernel.cpp:553-558) = sub[j % S2] *
-  // step[j / S2 + k*T/S2].
-  const float2 rsub = sub[j % S2];
-  gbyte *src = (gbyte *)p.station[st];
-  const int s0 = offset + j;
-
-  float2 v[32];
-  if constexpr (CPLX) {
-    // [Are, Aim, Bre, Bim] int8 per sample, one little-endian 32-bit word
-    gint *w = reinterpret_cast<gint *>(src + 4 * (size_t)s0);
-    const int sh = 16 * pol;
-#pragma unroll
-    for (int k = 0; k < 32; k++) {
-      const int wv = w[(size_t)k * T];
-      const float re = (float)(int8_t)((wv >> sh) & 0xff), im = (float)(int8_t)((wv >> (sh + 8)) & 0xff);
-      v[k] = cmul(cmul(rsub, step[j / S2 + k * (T / S2)]), make_float2(re, im));
-    }
-  } else {
-    // byte layout [A_t, B_t, A_t+1, B_t+1] (fxkernel.cpp:443-476): sample
-    // s is nibble s&1 of byte s>>1; pol A in the nibble's low 2 bits.
-    // T is even, so every k shares the nibble parity of s0.
-    gbyte *bsrc = src + (s0 >> 1);
-    const int sh = (s0 & 1) * 4 + 2 * pol;
-#pragma unroll
-    for (int k = 0; k < 32; k++) {
-      const int code = (bsrc[(size_t)k * (T / 2)] >> sh) & 3;
-      const float x = (code & 2) ? ((code & 1) ? kHiMag : 1.0f) : ((code & 1) ? -1.0f : -kHiMag);
-      const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
-      v[k] = make_float2(x * r.x, x * r.y);
-    }
-  }
-#pragma unroll
-  for (int bf = 0; bf < 32 / G::radix(0); bf++) dft<G::radix(0)>(&v[bf * G::radix(0)]);
-  later_passes<L, CPLX, 1>(v, splane, j, tw);
-
-  // last pass outputs: bins q = jj + k*(L/R), keep q < nchan
-  constexpr int PL = G::NPASS - 1;
-  constexpr int R = G::radix(PL), NB = 32 / R;
-  constexpr int KEEP = CPLX ? R : R / 2;
-#pragma unroll
-  for (int bf = 0; bf < NB; bf++) {
-    const int jj = j + bf * T;
-#pragma unroll
-    for (int k = 0; k < KEEP; k++) {
-      const int q = jj + k * (L / R);
-      const float2 fr = cmul(fsub[q % S], fstep[q / S]);  // fracsamprotator, fxkernel.cpp:614-625
-      store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, cmul(fr, v[bf * R + k]));
-    }
-  }
+  __shared__ float2 smem[BLds<L, CPLX>::N];
+  fstage_big_item<L, CPLX, OUT>(p, blockIdx.x, smem);  // grid = 2 * nant * nt exactly
 }
 
 template <int L, bool CPLX>
@@ -226,6 +37,9 @@ hipError_t launch_big(const FParams &p, hipStream_t s) {
 
 hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s) {
   switch (L) {
+    case 2048: return cplx ? launch_big<2048, true>(p, s) : launch_big<2048, false>(p, s);
+    case 4096: return cplx ? launch_big<4096, true>(p, s) : launch_big<4096, false>(p, s);
+    case 8192: return cplx ? launch_big<8192, true>(p, s) : launch_big<8192, false>(p, s);
     case 16384: return cplx ? launch_big<16384, true>(p, s) : launch_big<16384, false>(p, s);
     case 32768: return cplx ? launch_big<32768, true>(p, s) : launch_big<32768, false>(p, s);
     default: return hipErrorInvalidValue;

@@ -1,10 +1,12 @@
 // Multi-subband F-stage: all subbands of a subintegration in one persistent
 // launch.  Subbands differ in FFT length, bandwidth, delays and input; the
-// workgroup size is common (256, or 512 when an 8192-point subband is
-// present), so a unit of work is one workgroup's worth of items (station,
-// FFT) of one subband, block/(L/16) items at FFT length L.  Units are strided
+// workgroup size is common (256 or 512 threads), so a unit of work is one
+// workgroup's worth of items (station, FFT) of one subband, block/(L/16)
+// items at FFT length L, or at 256 threads and L = 8192 one (station, FFT,
+// polarisation) through the register/plane F-stage.  Units are strided
 // over a grid sized to fill the device once; each unit runs the same item
 // code as the per-subband launch (fstage_item.hpp).
+#include "fstage_big.hpp"
 #include "fstage_item.hpp"
 
 namespace gxc {
@@ -13,11 +15,12 @@ namespace {
 using namespace fs;
 
 // LDS of the largest unit of a launch with workgroups of BLOCK threads:
-// FFT lengths 256..8192 at 512 threads, 256..4096 at 256.
+// FFT lengths 256..8192 at 512 threads; at 256 threads 256..4096 with
+// fstage_item and 8192 one polarisation per unit with fstage_big_item.
 template <int BLOCK>
 struct MultiLds {
   static constexpr int max_n() {
-    int m = 0;
+    int m = BLOCK == 256 ? BLds<8192, false>::N : 0;
     for (int L = 256; L <= (BLOCK == 512 ? 8192 : 4096); L *= 2) {
       const int n = flds_slots(L, false, BLOCK);
       m = n > m ? n : m;
@@ -32,7 +35,10 @@ struct MultiLds {
 // register allocation instead of one merged over every length.
 template <int L, int OUT, int BLOCK>
 __device__ __attribute__((noinline)) void multi_unit(const FSeg *g, int blk, float2 *smem) {
-  fstage_item<L, false, OUT, BLOCK>(g->p, blk, smem);
+  if constexpr (L == 8192 && BLOCK == 256)
+    fstage_big_item<8192, false, OUT>(g->p, blk, smem);  // blk = 2 * item + pol
+  else
+    fstage_item<L, false, OUT, BLOCK>(g->p, blk, smem);
 }
 
 template <int OUT, int BLOCK>
@@ -51,9 +57,7 @@ __global__ void __launch_bounds__(BLOCK) fstage_multi_kernel(const FSeg *segs, i
       case 2048: multi_unit<2048, OUT, BLOCK>(&g, blk, smem); break;
       case 4096: multi_unit<4096, OUT, BLOCK>(&g, blk, smem); break;
       default:
-        if constexpr (BLOCK == 512) {
-          if (g.L == 8192) multi_unit<8192, OUT, BLOCK>(&g, blk, smem);
-        }
+        if (g.L == 8192) multi_unit<8192, OUT, BLOCK>(&g, blk, smem);
         break;
     }
   }
@@ -79,9 +83,15 @@ hipError_t launch_multi(const FSeg *segs, int nseg, long long total, int out_mod
 
 bool fstage_multi_supported(int L, bool cplx) { return !cplx && L >= 256 && L <= 8192 && (L & (L - 1)) == 0; }
 
-int fstage_multi_block(int maxL) { return maxL >= 8192 ? 512 : 256; }
+int fstage_multi_block(int maxL) {
+  // 256 covers every length (8192 as one-polarisation units), but those
+  // units then share the occupancy of the 4096-point items (2 workgroups
+  // per CU); 512 with whole 8192-point items measured faster.
+  return maxL >= 8192 ? 512 : 256;
+}
 
 long long fstage_multi_units(int L, int block, long long items) {
+  if (L == 8192 && block == 256) return 2 * items;
   const long long ipw = block / (L / 16);
   return (items + ipw - 1) / ipw;
 }

@@ -33,6 +33,7 @@ struct Sub {
   gxc::host::Derived d;
   int ntc = 0;     // FFTs of the largest chunk
   int splits = 1;  // X-stage FFT splits
+  int partial_cap = 0;  // splits the partial buffer holds
   int npow = 0;    // FFTs sampled for the scales
   // input
   bool input_set = false, input_device = false;
@@ -61,6 +62,7 @@ struct gxc_batch {
   std::vector<Sub> sub;
   std::string err;
   int device = 0, out_mode = 0, M = 0, nrows = 0, nchunks = 1, nbuf = 1, fblock = 256;
+  int maxL = 0, xunits = 2048;  // longest FFT; X units per launch the splits aim at
   hipStream_t stream = nullptr, xstream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_f[2] = {nullptr, nullptr}, ev_x[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec = nullptr;
@@ -171,6 +173,15 @@ int chunk_begin(const Sub &u, int c, int nchunks) {
 // Segment tables of every launch, from the handle's buffers (fixed for the
 // handle's lifetime: inputs and delays are reached through device arrays).
 gxc_status build_tables(gxc_batch *b) {
+  for (void *p : {(void *)b->d_fseg, (void *)b->d_xseg, (void *)b->d_countseg, (void *)b->d_finseg,
+                  (void *)b->d_powseg, (void *)b->d_scaleseg})
+    hipFree(p);
+  b->d_fseg = nullptr;
+  b->d_xseg = nullptr;
+  b->d_countseg = nullptr;
+  b->d_finseg = nullptr;
+  b->d_powseg = nullptr;
+  b->d_scaleseg = nullptr;
   const int ns = (int)b->sub.size(), C = b->nchunks;
   const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
   std::vector<gxc::FSeg> fseg((size_t)C * ns);
@@ -258,6 +269,31 @@ gxc_status build_tables(gxc_batch *b) {
     b->scaletotal = su;
     GXB_TRY(upload(b, &b->d_powseg, pseg));
     GXB_TRY(upload(b, &b->d_scaleseg, sseg));
+  }
+  return GXC_OK;
+}
+
+// X splits per subband so that a launch has about b->xunits units, each
+// covering a similar number of FFTs (a unit's X work is channels x FFTs x
+// M^2); partial buffers grow when needed.
+gxc_status assign_splits(gxc_batch *b) {
+  const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
+  const int tb = b->out_mode == gxc::OUT_F32 ? 16 : 32;
+  double work = 0;
+  for (const Sub &u : b->sub) work += (double)(u.cfg.nchan / ct) * u.ntc;
+  const double per_unit = std::max(1.0, work / b->xunits);
+  for (Sub &u : b->sub) {
+    const long long sp = std::llround(u.ntc / per_unit);
+    const long long maxs = std::max(1, u.ntc / (4 * tb));
+    u.splits = (int)std::max(1LL, std::min(std::min(sp, maxs), 256LL));
+    if (u.splits > u.partial_cap) {
+      hipFree(u.d_partial);
+      u.d_partial = nullptr;
+      u.partial_cap = 0;
+      GXB_TRY(dmalloc(b, reinterpret_cast<unsigned char **>(&u.d_partial),
+                      gxc::xm_partial_bytes(b->M, u.cfg.nchan, u.splits)));
+      u.partial_cap = u.splits;
+    }
   }
   return GXC_OK;
 }
@@ -486,20 +522,9 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
   gxc_status s = GXC_OK;
   if ((s = upload(b, &b->d_rowmap, gxc::host::make_rowmap(nant, cfg->autos != 0))) != GXC_OK) return fail(s);
   size_t spec_bytes = 0;
-  // X splits: about 2048 units over all subbands, each unit a similar
-  // number of FFTs (the X work of a unit is channels x FFTs x M^2).
-  const int ct = gxc::xm_channels_per_group(M, out_mode);
-  const int tb = out_mode == gxc::OUT_F32 ? 16 : 32;
-  double work = 0;
+  b->maxL = maxL;
+  for (Sub &u : b->sub) u.ntc = (u.cfg.numffts + C - 1) / C;
   for (Sub &u : b->sub) {
-    u.ntc = (u.cfg.numffts + C - 1) / C;
-    work += (double)(u.cfg.nchan / ct) * u.ntc;
-  }
-  const double per_unit = std::max(1.0, work / 2048.0);
-  for (Sub &u : b->sub) {
-    long long sp = std::llround(u.ntc / per_unit);
-    const long long maxs = std::max(1, u.ntc / (4 * tb));
-    u.splits = (int)std::max(1LL, std::min(std::min(sp, maxs), 256LL));
     u.npow = std::min(u.cfg.numffts, gxc::kPowerRows);
     u.spec_off = spec_bytes;
     spec_bytes += ((size_t)u.ntc * M * u.cfg.nchan * eb + 255) / 256 * 256;
@@ -515,9 +540,6 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
     if ((s = dmalloc(b, &u.d_vis, (size_t)b->nrows * 4 * u.cfg.nchan)) != GXC_OK) return fail(s);
     if (hipMemset(u.d_vis, 0, (size_t)b->nrows * 4 * u.cfg.nchan * sizeof(float2)) != hipSuccess)
       return fail(set_err(b, GXC_ERR_HIP, "hipMemset failed"));
-    if ((s = dmalloc(b, reinterpret_cast<unsigned char **>(&u.d_partial),
-                     gxc::xm_partial_bytes(M, u.cfg.nchan, u.splits))) != GXC_OK)
-      return fail(s);
     if (out_mode != gxc::OUT_F32) {
       if ((s = dmalloc(b, &u.d_scales, (size_t)M * u.cfg.nchan)) != GXC_OK) return fail(s);
       if ((s = dmalloc(b, &u.d_power, (size_t)u.npow * M * u.cfg.nchan)) != GXC_OK) return fail(s);
@@ -525,6 +547,7 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
   }
   for (int k = 0; k < b->nbuf; k++)
     if ((s = dmalloc(b, reinterpret_cast<unsigned char **>(&b->d_spec[k]), spec_bytes)) != GXC_OK) return fail(s);
+  if ((s = assign_splits(b)) != GXC_OK) return fail(s);
   if ((s = build_tables(b)) != GXC_OK) return fail(s);
   *out = b;
   return GXC_OK;
@@ -648,6 +671,101 @@ gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n
 gxc_status gxc_batch_set_profiling(gxc_batch *b, int enable) {
   if (!b) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null batch");
   b->profiling = enable != 0;
+  return GXC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Median of 3 timed subintegrations (ms) after one untimed one.
+gxc_status time_process(gxc_batch *b, double *out) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  GXB_HIP(b, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return set_err(b, GXC_ERR_HIP, "hipEventCreate failed");
+  }
+  gxc_status st = enqueue_process(b);
+  std::vector<float> ts;
+  for (int r = 0; r < 3 && st == GXC_OK; r++) {
+    float ms = 0.f;
+    hipEventRecord(e0, b->stream);
+    st = enqueue_process(b);
+    if (st == GXC_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = set_err(b, GXC_ERR_HIP, "event timing failed");
+    ts.push_back(ms);
+  }
+  if (st == GXC_OK && hipStreamSynchronize(b->stream) != hipSuccess) st = set_err(b, GXC_ERR_HIP, "sync failed");
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st != GXC_OK) return st;
+  std::sort(ts.begin(), ts.end());
+  *out = ts[ts.size() / 2];
+  return GXC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
+  if (!b) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null batch");
+  for (const Sub &u : b->sub)
+    if (!u.input_set || !u.delays_set) return set_err(b, GXC_ERR_STATE, "input and delays must be set");
+  GXB_HIP(b, hipStreamSynchronize(b->stream));
+  const bool prof = b->profiling;
+  b->profiling = false;
+  const int graph_cfg = b->cfg.graph;
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  b->graph_exec = nullptr;
+  b->cfg.graph = 0;
+  gxc_tuning t{};
+  int cand = 0;
+  double def_ms = 0, best = 1e30;
+  gxc_status st = time_process(b, &def_ms);
+  cand++;
+  int best_fb = b->fblock, best_units = b->xunits;
+  for (int fb : {256, 512}) {
+    if (st != GXC_OK) continue;
+    for (int units : {1024, 2048, 4096, 8192}) {
+      b->fblock = fb;
+      b->xunits = units;
+      st = assign_splits(b);
+      if (st == GXC_OK) st = build_tables(b);
+      double ms = 0;
+      if (st == GXC_OK) st = time_process(b, &ms);
+      cand++;
+      if (st != GXC_OK) break;
+      if (ms < best) best = ms, best_fb = fb, best_units = units;
+    }
+  }
+  if (st == GXC_OK) {
+    b->fblock = best_fb;
+    b->xunits = best_units;
+    st = assign_splits(b);
+    if (st == GXC_OK) st = build_tables(b);
+  }
+  double graph_ms = 1e30;
+  if (st == GXC_OK) {
+    b->cfg.graph = 1;
+    st = time_process(b, &graph_ms);
+    cand++;
+  }
+  b->cfg.graph = st == GXC_OK ? (graph_ms <= best ? 1 : 0) : graph_cfg;
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  b->graph_exec = nullptr;
+  b->profiling = prof;
+  if (st != GXC_OK) return st;
+  t.fblock = b->fblock;
+  t.xtile = gxc::xm_channels_per_group(b->M, b->out_mode);
+  t.xsplits = b->xunits;  // for a batch: the X units per launch the splits aim at
+  t.graph = b->cfg.graph;
+  t.process_ms = std::min(best, graph_ms);
+  t.default_ms = def_ms;
+  t.candidates = cand;
+  if (out) *out = t;
   return GXC_OK;
 }
 

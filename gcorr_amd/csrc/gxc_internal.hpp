@@ -83,12 +83,15 @@ long long persistent_grid(const void *kernel, int block, size_t dyn_lds, long lo
 
 // Launchers (defined in the .hip files).  Return hipSuccess or the launch error.
 bool fstage_supported(int L, bool cplx);
-hipError_t launch_fstage(const FParams &p, int L, bool cplx, hipStream_t s);      // any supported L
+// F-stage workgroup sizes available at FFT length L (default first).
+int fstage_blocks(int L, bool cplx, int *out /* [4] */);
+// block: one of fstage_blocks (0 = default); ignored for L > 8192
+hipError_t launch_fstage(const FParams &p, int L, bool cplx, int block, hipStream_t s);
 hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s);  // L = 16384, 32768
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
 // Multi-subband F-stage: real 2-bit input, FFT lengths 256..8192, workgroups
-// of `block` threads (256, or 512 when a subband has L = 8192); units of a
-// subband = ceil(nant * nt / (block / (L/16))).
+// of `block` threads (256, the default, or 512); units of a subband =
+// ceil(nant * nt / (block / (L/16))), or 2 * nant * nt at L = 8192 and 256.
 bool fstage_multi_supported(int L, bool cplx);
 int fstage_multi_block(int maxL);
 long long fstage_multi_units(int L, int block, long long items);

@@ -52,7 +52,10 @@ int xm_choose_splits(int M, int nchan, int nt, int out_mode);
 size_t xm_partial_bytes(int M, int nchan, int splits);
 int xm_element_bytes(int out_mode);   // bytes of one packed complex value
 
-hipError_t launch_xmfma(const XMParams &p, int out_mode, hipStream_t s);
+// ct: channel tile, one of xm_tiles (0 = default)
+hipError_t launch_xmfma(const XMParams &p, int out_mode, int ct, hipStream_t s);
+// Channel tiles available for M inputs (default first); returns the count (<= 3).
+int xm_tiles(int M, int out_mode, int *out);
 // scales[i] = power of two nearest target / sqrt(mean_r power[r][i]), r < nrows.
 hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s);
 constexpr int kPowerRows = 64;  // FFTs sampled for the scale estimate

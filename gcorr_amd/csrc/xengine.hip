@@ -34,7 +34,11 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int ET, int MB>
+// Default channel tile: 16 (FP32) or 32 channels per workgroup over the
+// input blocks.
+constexpr int xm_default_ct(int et, int mb) { return (et == OUT_F32 ? 16 : 32) / mb; }
+
+template <int ET, int MB, int CT_ = xm_default_ct(ET, MB)>
 struct XM {
   static constexpr bool F32 = ET == OUT_F32;
   static constexpr int EB = F32 ? 4 : ET == OUT_FP8 ? 1 : 2;  // bytes per real component
@@ -42,7 +46,7 @@ struct XM {
   static constexpr int K = F32 ? 4 : 32;                     // FFTs per MFMA
   static constexpr int TB = F32 ? 16 : 32;                   // FFTs per LDS stage
   static constexpr int MP = 16 * MB;                         // inputs padded to blocks
-  static constexpr int CT = (F32 ? 16 : 32) / MB;            // channels per workgroup
+  static constexpr int CT = CT_;                             // channels per workgroup
   static constexpr int NTILE = MB * (MB + 1) / 2;
   static constexpr int EPL = 16 / CB;  // complex values (channels) per 16-byte load
   static constexpr int CPW = CT / 4;   // channels per wave
@@ -96,10 +100,10 @@ struct Operand {
 // it transposed into LDS.  Tasks are strided over the 256 threads; in a
 // task the channel column is the fastest index, so neighbouring lanes load
 // neighbouring 16-byte columns of one spectra row.
-template <int ET, int MB>
+template <int ET, int MB, int CT>
 __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned char *src, size_t rowbytes,
                                            int M, int c0, int ts, int te, int tid) {
-  using X = XM<ET, MB>;
+  using X = XM<ET, MB, CT>;
   constexpr int NQ = X::CT / X::EPL;  // 16-byte columns per row
   if constexpr (X::F32) {
     // task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads in flight,
@@ -189,9 +193,9 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
 // Channels group*CT .. group*CT+CT-1, FFT split `split`, on a workgroup of
 // 256 threads (every thread calls it with the same arguments: it contains
 // __syncthreads).  lds holds XM::LDSB bytes.
-template <int ET, int MB>
+template <int ET, int MB, int CT>
 __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int split, unsigned char *lds) {
-  using X = XM<ET, MB>;
+  using X = XM<ET, MB, CT>;
   using Op = typename Operand<ET>::type;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c0 = group * X::CT;
@@ -211,7 +215,7 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   const size_t rowbytes = (size_t)p.nchan * X::CB;
   for (int ts = tb; ts < te; ts += X::TB) {
     __syncthreads();  // the previous stage's operand reads are done
-    stage_tile<ET, MB>(lds, src, rowbytes, M, c0, ts, te, tid);
+    stage_tile<ET, MB, CT>(lds, src, rowbytes, M, c0, ts, te, tid);
     __syncthreads();
     // Wave w: channels w, w+4, ...  Lane l's operand: input l%16 of a block
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
@@ -293,12 +297,12 @@ __device__ __forceinline__ long long xcd_swizzle(long long u, long long total) {
   return (u % kXcds) * per + u / kXcds;
 }
 
-template <int ET, int MB>
+template <int ET, int MB, int CT>
 __global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  const int groups = p.nchan / XM<ET, MB>::CT;
+  const int groups = p.nchan / CT;
   const long long u = xcd_swizzle(blockIdx.x, (long long)groups * p.splits);
-  xmfma_item<ET, MB>(p, (int)(u % groups), (int)(u / groups), lds);
+  xmfma_item<ET, MB, CT>(p, (int)(u % groups), (int)(u / groups), lds);
 }
 
 // Segment of a multi-subband launch holding unit u (segments in order).
@@ -320,9 +324,10 @@ __global__ void __launch_bounds__(256) xmfma_multi_kernel(const XSeg *segs, int 
     const long long u = xcd_swizzle(v, total);
     const XSeg &g = segs[find_seg(segs, nseg, u)];
     const int local = (int)(u - g.unit0);
-    const int groups = g.p.nchan / XM<ET, MB>::CT;
+    constexpr int CT = xm_default_ct(ET, MB);
+    const int groups = g.p.nchan / CT;
     __syncthreads();  // the previous unit's LDS reads are done
-    xmfma_item<ET, MB>(g.p, local % groups, local / groups, lds);
+    xmfma_item<ET, MB, CT>(g.p, local % groups, local / groups, lds);
   }
 }
 
@@ -444,22 +449,40 @@ __global__ void quantise_kernel(void *out, const float2 *in, size_t n, int mode)
   }
 }
 
-template <int ET, int MB>
+template <int ET, int MB, int CT>
 hipError_t launch_xm(const XMParams &p, hipStream_t s) {
-  using X = XM<ET, MB>;
+  using X = XM<ET, MB, CT>;
   if (p.nchan % X::CT) return hipErrorInvalidValue;
   const long long units = (long long)(p.nchan / X::CT) * p.splits;
   if (units > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 grid((unsigned)units);
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be allowed once per kernel
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB, CT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((xmfma_kernel<ET, MB>), grid, dim3(256), X::LDSB, s, p);
+  hipLaunchKernelGGL((xmfma_kernel<ET, MB, CT>), grid, dim3(256), X::LDSB, s, p);
   return hipGetLastError();
+}
+
+// Channel tiles offered besides the default: half (fewer accumulators,
+// more workgroups) and double (longer rows per load), where the tile holds
+// whole 16-byte columns, every wave has a channel, the accumulators of a
+// wave stay within 192 registers and the LDS tile within 160 KiB.
+template <int ET, int MB, int CT>
+constexpr bool xm_tile_ok() {
+  using X = XM<ET, MB, CT>;
+  return CT >= 4 && CT % X::EPL == 0 && (CT / 4) * X::NTILE * 12 <= 192 && X::LDSB <= 160 * 1024;
+}
+
+template <int ET, int MB, int CT>
+hipError_t launch_xm_if(const XMParams &p, hipStream_t s) {
+  if constexpr (xm_tile_ok<ET, MB, CT>())
+    return launch_xm<ET, MB, CT>(p, s);
+  else
+    return hipErrorInvalidValue;
 }
 
 template <int ET, int MB>
@@ -488,13 +511,42 @@ hipError_t launch_xm_multi_mb(const XSeg *segs, int nseg, long long total, int M
   }
 }
 
+template <int ET, int MB>
+hipError_t launch_xm_ct(const XMParams &p, int ct, hipStream_t s) {
+  constexpr int D = xm_default_ct(ET, MB);
+  if (ct == 0 || ct == D) return launch_xm<ET, MB, D>(p, s);
+  if (ct == D / 2) return launch_xm_if<ET, MB, D / 2>(p, s);
+  if (ct == 2 * D) return launch_xm_if<ET, MB, 2 * D>(p, s);
+  return hipErrorInvalidValue;
+}
+
 template <int ET>
-hipError_t launch_xm_mb(const XMParams &p, hipStream_t s) {
+hipError_t launch_xm_mb(const XMParams &p, int ct, hipStream_t s) {
   switch (xm_blocks(p.M)) {
-    case 1: return launch_xm<ET, 1>(p, s);
-    case 2: return launch_xm<ET, 2>(p, s);
-    case 4: return launch_xm<ET, 4>(p, s);
+    case 1: return launch_xm_ct<ET, 1>(p, ct, s);
+    case 2: return launch_xm_ct<ET, 2>(p, ct, s);
+    case 4: return launch_xm_ct<ET, 4>(p, ct, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+template <int ET, int MB>
+int tiles_of(int *out) {
+  constexpr int D = xm_default_ct(ET, MB);
+  int n = 0;
+  out[n++] = D;
+  if constexpr (xm_tile_ok<ET, MB, D / 2>()) out[n++] = D / 2;
+  if constexpr (xm_tile_ok<ET, MB, 2 * D>()) out[n++] = 2 * D;
+  return n;
+}
+
+template <int ET>
+int tiles_mb(int M, int *out) {
+  switch (xm_blocks(M)) {
+    case 1: return tiles_of<ET, 1>(out);
+    case 2: return tiles_of<ET, 2>(out);
+    case 4: return tiles_of<ET, 4>(out);
+    default: return 0;
   }
 }
 
@@ -538,13 +590,23 @@ float xm_target_rms(int out_mode) {
   return out_mode == OUT_FP8 ? 32.0f : 1.0f;
 }
 
-hipError_t launch_xmfma(const XMParams &p, int out_mode, hipStream_t s) {
+hipError_t launch_xmfma(const XMParams &p, int out_mode, int ct, hipStream_t s) {
   switch (out_mode) {
-    case OUT_F32: return launch_xm_mb<OUT_F32>(p, s);
-    case OUT_F16: return launch_xm_mb<OUT_F16>(p, s);
-    case OUT_BF16: return launch_xm_mb<OUT_BF16>(p, s);
-    case OUT_FP8: return launch_xm_mb<OUT_FP8>(p, s);
+    case OUT_F32: return launch_xm_mb<OUT_F32>(p, ct, s);
+    case OUT_F16: return launch_xm_mb<OUT_F16>(p, ct, s);
+    case OUT_BF16: return launch_xm_mb<OUT_BF16>(p, ct, s);
+    case OUT_FP8: return launch_xm_mb<OUT_FP8>(p, ct, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+int xm_tiles(int M, int out_mode, int *out) {
+  switch (out_mode) {
+    case OUT_F32: return tiles_mb<OUT_F32>(M, out);
+    case OUT_F16: return tiles_mb<OUT_F16>(M, out);
+    case OUT_BF16: return tiles_mb<OUT_BF16>(M, out);
+    case OUT_FP8: return tiles_mb<OUT_FP8>(M, out);
+    default: return 0;
   }
 }
 

@@ -16,7 +16,7 @@ from ._lib import check
 
 
 def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact_phase,
-            chunk_ffts, overlap=False, graph=True):
+            chunk_ffts, overlap=False, graph=True, fblock=0, xtile=0, xsplits=0):
     c = _lib.Config()
     _lib.load().gxc_config_init(ctypes.byref(c))
     c.nant, c.nchan, c.numffts, c.nbit = int(nant), int(nchan), int(nfft), int(nbit)
@@ -29,6 +29,7 @@ def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact
     c.chunk_ffts = int(chunk_ffts)
     c.overlap = int(bool(overlap))
     c.graph = int(bool(graph))
+    c.fblock, c.xtile, c.xsplits = int(fblock), int(xtile), int(xsplits)
     return c
 
 
@@ -46,10 +47,11 @@ class FxKernel:
     """FxKernel(nant, nchan, nfft, nbit, lo, bw, is_complex) on the MI355X path."""
 
     def __init__(self, nant, nchan, nfft, nbit, lo, bw, is_complex=False, *, precision="fp32",
-                 autos=False, exact_phase=False, chunk_ffts=0, overlap=False, graph=True):
+                 autos=False, exact_phase=False, chunk_ffts=0, overlap=False, graph=True,
+                 fblock=0, xtile=0, xsplits=0):
         self._L = _lib.load()
         self.cfg = _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos,
-                           exact_phase, chunk_ffts, overlap, graph)
+                           exact_phase, chunk_ffts, overlap, graph, fblock, xtile, xsplits)
         self.nant, self.nchan, self.numffts = int(nant), int(nchan), int(nfft)
         self.nrows = int(self._L.gxc_num_rows(ctypes.byref(self.cfg)))
         self.station_bytes = int(self._L.gxc_station_bytes(ctypes.byref(self.cfg)))
@@ -156,6 +158,13 @@ class FxKernel:
         check(self._L.gxc_fstage(self._h, int(t0), int(nfft), spec.ctypes.data_as(ctypes.c_void_p),
                                  valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), self._h)
         return spec, valid.astype(bool)
+
+    def autotune(self):
+        """gxc_autotune: time the launch shapes on this handle's input and keep
+        the fastest.  Returns the choice and its timings as a dict."""
+        t = _lib.Tuning()
+        check(self._L.gxc_autotune(self._h, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
     def set_profiling(self, on=True):
         check(self._L.gxc_set_profiling(self._h, int(bool(on))), self._h)

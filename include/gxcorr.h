@@ -85,7 +85,12 @@ typedef struct {
                       spectra); 0: one stream, one chunk after another */
   int graph;       /* 1: capture the device work of one subintegration as a
                       hipGraph on first use and replay it afterwards */
-  int reserved[4];
+  /* Launch shapes (0 = default; gxc_autotune measures and sets them): */
+  int fblock;      /* F-stage workgroup size, i.e. FFT blocks per workgroup
+                      = fblock / (fftchannels/16) */
+  int xtile;       /* X-stage channel tile (channels per workgroup) */
+  int xsplits;     /* X-stage FFT splits (workgroups per channel tile) */
+  int reserved[1];
 } gxc_config;
 
 typedef struct gxc_handle gxc_handle;
@@ -184,6 +189,23 @@ gxc_status gxc_fstage(gxc_handle *h, int t0, int nfft, gxc_cf32 *spectra,
 gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft,
                        int autos, int precision, gxc_cf32 *out);
 
+/* Autotuner: with input and delays set, times the F-stage at each available
+ * workgroup size (FFT blocks per workgroup), the X-stage at each channel
+ * tile and split count, and whole subintegrations with and without graph
+ * replay, on this handle's own data; keeps the fastest.  The choice is
+ * returned (may be NULL) and applies to later gxc_process calls; it can be
+ * given back through gxc_config.fblock/xtile/xsplits/graph of a new handle
+ * with the same shape to skip the search.  No visibilities are kept from
+ * the trial runs: call gxc_process afterwards. */
+typedef struct {
+  int fblock, xtile, xsplits, graph;  /* the choice */
+  double fstage_ms, xstage_ms;        /* their times (all chunks of one subint) */
+  double process_ms;                  /* one subintegration, as chosen */
+  double default_ms;                  /* one subintegration at the defaults */
+  int candidates;                     /* configurations timed */
+} gxc_tuning;
+gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out);
+
 /* Per-stage timing (see gxc_timings).  Enabled by gxc_set_profiling. */
 gxc_status gxc_set_profiling(gxc_handle *h, int enable);
 gxc_status gxc_get_timings(const gxc_handle *h, gxc_timings *t);
@@ -244,6 +266,13 @@ gxc_status gxc_batch_synchronize(gxc_batch *b);
 /* Subband `sub`'s visibilities [rows][4][nchan] and row counts. */
 gxc_status gxc_batch_get_visibilities(gxc_batch *b, int sub, gxc_cf32 *out, size_t nelem);
 gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n);
+/* Autotuner of a batch (input and delays of every subband set): times whole
+ * subintegrations for each F-stage workgroup size (256/512 threads) and X
+ * unit count (1024..8192, which sets every subband's FFT splits), then with
+ * and without graph replay, and keeps the fastest.  In the result, xsplits
+ * holds the chosen X unit count and fstage_ms/xstage_ms are 0. */
+gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out);
+
 /* Timing over all subbands (launch counts are per-stage launches, one per
  * chunk for F and X). */
 gxc_status gxc_batch_set_profiling(gxc_batch *b, int enable);

@@ -163,3 +163,10 @@ def test_batch_create_checks_config_before_device():
     assert create(ok) == 3                                     # valid, but no GPU here
     msg = L.gxc_batch_last_error(None).decode()
     assert "device" in msg
+
+
+def test_autotune_null_handle_is_an_error():
+    L = _lib.load()
+    t = _lib.Tuning()
+    assert L.gxc_autotune(None, ctypes.byref(t)) == 1
+    assert L.gxc_batch_autotune(None, ctypes.byref(t)) == 1

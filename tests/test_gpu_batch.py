@@ -118,3 +118,21 @@ def test_batch_graph_replays_with_new_delays_and_device_input(gpu):
             np.testing.assert_array_equal(b.visibilities(s), eager[s][0])
             np.testing.assert_array_equal(b.row_counts(s), eager[s][1])
     b.close()
+
+
+def test_batch_autotune_keeps_results(gpu):
+    ws = subbands(div=500)
+    datas = random_inputs(ws, seed=6)
+    delays = [synth.make_delays(w) for w in ws]
+    b = gcorr_amd.Batch(8, [(w.nchan, w.numffts, w.lo, w.bw) for w in ws], autos=True)
+    for s in range(len(ws)):
+        b.set_input(s, datas[s])
+        b.set_delays(s, *delays[s])
+    t = b.autotune()
+    assert t["candidates"] >= 5 and t["fblock"] in (256, 512) and t["process_ms"] > 0
+    b.process()
+    for s, w in enumerate(ws):
+        ref, rc = oracle(w, datas[s], *delays[s])
+        np.testing.assert_array_equal(b.row_counts(s), rc)
+        assert eps(b.visibilities(s), ref) < EPS
+    b.close()

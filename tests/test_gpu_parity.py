@@ -363,3 +363,75 @@ def test_fstage_fft_matches_rocfft(gpu, nchan, is_complex):
         got = spec[:, 2 * a:2 * a + 2]
         scale = np.sqrt(np.mean(np.abs(ref) ** 2))
         assert np.max(np.abs(got - ref)) / scale < 2e-5, a
+
+
+# ------------------------------------------------------------------ autotuner
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_autotune_keeps_parity_and_reports_choice(gpu, precision):
+    """The autotuner only changes launch shapes: after it, results still
+    match the oracle (FP32), or the untuned handle within FP16 resolution
+    (another F-stage kernel rounds the FP32 spectra differently, which moves
+    some FP16 roundings by one unit: 2^-11 relative on single products)."""
+    w, data, poly, offs = random_case(6, 1024, 400, seed=13)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True,
+                           precision=precision)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    before = k.visibilities()
+    t = k.autotune()
+    assert t["candidates"] >= 5 and t["fblock"] > 0 and t["xtile"] > 0 and t["xsplits"] >= 1
+    assert t["process_ms"] > 0 and t["default_ms"] > 0
+    k.process()
+    after = k.visibilities()
+    if precision == "fp32":
+        ref, _ = run_oracle(w, data, poly, offs)
+        assert eps(after, ref) < EPS
+    assert eps(after, before) < (1e-5 if precision == "fp32" else 1e-4)
+    # the choice handed to a new handle through the config gives the same result
+    k2 = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True,
+                            precision=precision, fblock=t["fblock"], xtile=t["xtile"],
+                            xsplits=t["xsplits"], graph=bool(t["graph"]))
+    k2.setInputData(data)
+    k2.setDelays(poly, offs)
+    k2.process()
+    np.testing.assert_array_equal(k2.visibilities(), after)
+    k.close()
+    k2.close()
+
+
+@pytest.mark.parametrize("fblock_mult,xtile_mult", [(0.5, 1), (2, 1), (1, 0.5), (1, 2)])
+def test_launch_shape_variants_match_oracle(gpu, fblock_mult, xtile_mult):
+    """Every F workgroup size and X channel tile the autotuner can pick."""
+    w, data, poly, offs = random_case(5, 512, 150, seed=14)
+    fb = int(256 * fblock_mult)        # default for L = 1024 is 256 threads
+    xt = int(16 * xtile_mult)          # default FP32 tile at M <= 16 is 16 channels
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True,
+                           fblock=fb, xtile=xt)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(k.row_counts(), rc)
+    assert eps(k.visibilities(), ref) < EPS
+    k.close()
+
+
+@pytest.mark.parametrize("nchan,is_complex", [(1024, False), (2048, False), (4096, False),
+                                              (2048, True), (4096, True), (8192, True)])
+def test_long_fft_kernel_at_short_lengths(gpu, nchan, is_complex):
+    """At FFT lengths 2048..8192 the one-polarisation-per-workgroup kernel
+    (fblock = L/32) is an alternative the autotuner can choose."""
+    w, data, poly, offs = random_case(3, nchan, 40, seed=15 + nchan, is_complex=is_complex,
+                                      bw=128e6 if is_complex else 64e6)
+    L = w.fftchannels
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                           autos=True, fblock=L // 32)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    ref, rc = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(k.row_counts(), rc)
+    assert eps(k.visibilities(), ref) < EPS
+    k.close()
