@@ -12,7 +12,10 @@ using namespace fs;
 template <int L, bool CPLX, int OUT>
 __global__ void __launch_bounds__((BG<L, CPLX>::T)) fstage_big_kernel(FParams p) {
   __shared__ float2 smem[BLds<L, CPLX>::N];
-  fstage_big_item<L, CPLX, OUT>(p, blockIdx.x, smem);  // grid = 2 * nant * nt exactly
+  // grid = 2 * nant * nt exactly; the two polarisations of one (station,
+  // FFT) are consecutive units on one XCD, so the second finds the input
+  // words in L2
+  fstage_big_item<L, CPLX, OUT>(p, (int)xcd_order(blockIdx.x, gridDim.x), smem);
 }
 
 template <int L, bool CPLX>

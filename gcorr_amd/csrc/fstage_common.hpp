@@ -1,5 +1,6 @@
-// Device helpers shared by the F-stage kernels (fstage.hip: FFTs held in LDS
-// with both polarisations, L <= 8192; fstage_big.hip: L = 16384, 32768).
+// Device helpers shared by the F-stage kernels: fstage_item.hpp (FFT held in
+// LDS with both polarisations, L <= 8192) and fstage_big.hpp (one
+// polarisation per workgroup, L = 2048..32768).
 #pragma once
 
 #include "gxc_internal.hpp"
@@ -269,6 +270,17 @@ __device__ __forceinline__ void store_bin(const FParams &p, size_t row, int srow
       static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_bf16(y.x * sc, y.y * sc);
     }
   }
+}
+
+// Workgroups are dealt to the 8 XCDs round robin in dispatch order, each XCD
+// with its own L2.  Renumbering unit v of `total` so that consecutive units
+// run on one XCD one after another lets neighbouring units share lines in
+// that L2 (the tail beyond a multiple of 8 keeps its number).
+constexpr int kXcdCount = 8;
+__device__ __forceinline__ long long xcd_order(long long v, long long total) {
+  const long long per = total / kXcdCount;
+  if (v >= per * kXcdCount) return v;
+  return (v % kXcdCount) * per + v / kXcdCount;
 }
 
 // Global-address-space view of the packed station buffers: the loads then

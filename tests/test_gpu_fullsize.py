@@ -1,0 +1,103 @@
+"""The workloads at their full sizes, checked through properties that do not
+depend on size (the oracle is checked at reduced FFT counts elsewhere).
+
+Input: uniformly random packed bytes generated on the device, so every
+2-bit code (or int8 value) is equally likely and stations are independent.
+Properties:
+  - autocorrelations equal the closed form E|X_c|^2 = L * E[x^2] per
+    channel, within the sampling error over the subintegration's FFTs
+    (real 2-bit: E[x^2] = (2 * 3.3359^2 + 2) / 4; complex int8 uniform on
+    [-128, 127]: E|x|^2 = 2 * (128^2 + ... ) / 256, computed below);
+  - Hermitian products: V_BA of an auto row is conj(V_AB);
+  - cross correlations of independent stations are noise of the size the
+    closed form predicts (|V| ~ sqrt(P_j P_k / N));
+  - the FP16 form agrees with the FP32 form within 1e-3, and repeated runs
+    are identical bit for bit.
+"""
+import numpy as np
+import pytest
+
+import gcorr_amd
+from gcorr_amd import metrics, synth
+
+pytestmark = pytest.mark.gpu
+
+HIMAG = 3.3359
+
+
+def device_input(w, seed):
+    import torch
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(seed)
+    bufs = [torch.randint(0, 256, (w.station_bytes,), dtype=torch.uint8, device="cuda:0", generator=g)
+            for _ in range(w.nant)]
+    torch.cuda.synchronize()
+    return bufs
+
+
+def run(w, bufs, poly, offs, precision):
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                           precision=precision, autos=True)
+    k.setInputDevice([b.data_ptr() for b in bufs], keepalive=bufs)
+    k.setDelays(poly, offs)
+    k.process()
+    v, c = k.visibilities(), k.row_counts()
+    k.process()
+    v2 = k.visibilities()
+    k.close()
+    np.testing.assert_array_equal(v, v2)  # deterministic
+    return v, c
+
+
+def check_properties(w, v, c):
+    nbl = w.nant * (w.nant - 1) // 2
+    L = w.fftchannels
+    if w.is_complex:
+        vals = np.arange(-128, 128, dtype=np.float64)
+        ex2 = 2 * np.mean(vals ** 2)
+    else:
+        ex2 = (2 * HIMAG ** 2 + 2) / 4
+    power = L * ex2
+    autos = v[nbl:]
+    n = c[nbl:].min()
+    # AA and BB: real, positive, mean over channels within 6 sigma of the
+    # closed form (each channel is a mean of n exponential-like variates)
+    for p in (0, 3):
+        a = autos[:, p]
+        assert np.all(np.abs(a.imag) <= 1e-6 * np.abs(a.real) + 1e-3)
+        rel = a.real.mean(axis=1) / power - 1
+        assert np.all(np.abs(rel) < 6 / np.sqrt(n * w.nchan)), rel
+    # Hermitian: BA = conj(AB) exactly (same sums, Im from the antisymmetric part)
+    np.testing.assert_allclose(autos[:, 2], np.conj(autos[:, 1]), rtol=1e-5, atol=1e-3 * power / np.sqrt(n))
+    # independent stations: cross power ~ P^2 / n per channel
+    cross = v[:nbl]
+    ms = np.mean(np.abs(cross) ** 2)
+    assert 0.5 < ms / (power ** 2 / n) < 2.0, ms / (power ** 2 / n)
+
+
+@pytest.mark.parametrize("key", ["c2", "c3"])
+def test_full_size_real_2bit(gpu, key):
+    w = synth.WORKLOADS[key]
+    bufs = device_input(w, seed=21)
+    poly, offs = synth.make_delays(w)
+    v32, c = run(w, bufs, poly, offs, "fp32")
+    check_properties(w, v32, c)
+    v16, c16 = run(w, bufs, poly, offs, "fp16")
+    np.testing.assert_array_equal(c16, c)
+    assert metrics.eps(v16, v32) < 1e-3
+
+
+def test_full_size_complex8_32_stations(gpu):
+    """32 stations x 32768 channels, 15 625 FFTs of complex int8 (65.5 GB of
+    input in HBM): FP8 e4m3 and FP16 forms against FP32."""
+    w = synth.WORKLOADS["c5"]
+    bufs = device_input(w, seed=22)
+    poly, offs = synth.make_delays(w)
+    v32, c = run(w, bufs, poly, offs, "fp32")
+    check_properties(w, v32, c)
+    v16, _ = run(w, bufs, poly, offs, "fp16")
+    assert metrics.eps(v16, v32) < 1e-3
+    v8, _ = run(w, bufs, poly, offs, "fp8")
+    e8 = metrics.eps(v8, v32)
+    print("c5 full size: fp16 eps %.3g fp8 eps %.3g" % (metrics.eps(v16, v32), e8))
+    assert e8 < 0.12
