@@ -118,7 +118,7 @@ __device__ __forceinline__ void later_passes(float2 (&v)[32], float *s, int j, c
 template <int L, bool CPLX>
 struct BLds {
   static constexpr int PLANE2 = (BG<L, CPLX>::PADL + 1) / 2;  // plane, in float2 slots
-  static constexpr int N = PLANE2 + TabGeom<L, CPLX>::NTAB;
+  static constexpr int N = PLANE2 + TabGeom<L, CPLX>::NTAB + 2;  // + 4-float 2-bit LUT
 };
 
 // Unit u = 2 * (station + nant * FFT) + pol of one launch, on a workgroup
@@ -153,6 +153,10 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
     return;
   }
   fill_tables<L, CPLX>(p, frac, a, b, stab, j, T);
+  // 2-bit levels by code (fxkernel.cpp:15-28): an LDS read replaces the
+  // compare/select chain that would otherwise cost ~6 VALU per sample
+  float *lut = reinterpret_cast<float *>(stab + TabGeom<L, CPLX>::NTAB);
+  if (!CPLX && j < 4) lut[j] = j == 0 ? -kHiMag : j == 1 ? -1.0f : j == 2 ? 1.0f : kHiMag;
   __syncthreads();
   const float2 *sub = stab;
   const float2 *step = sub + S2;
@@ -187,7 +191,7 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
 #pragma unroll
     for (int k = 0; k < 32; k++) {
       const int code = (bsrc[(size_t)k * (T / 2)] >> sh) & 3;
-      const float x = (code & 2) ? ((code & 1) ? kHiMag : 1.0f) : ((code & 1) ? -1.0f : -kHiMag);
+      const float x = lut[code];
       const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
       v[k] = make_float2(x * r.x, x * r.y);
     }

@@ -190,7 +190,13 @@ __device__ __forceinline__ bool split_offset(const FParams &p, int st, int t, in
 // Entries e = e0, e0+estep, ... of one item's rotator tables: sub, step
 // (fringe rotation, fxkernel.cpp:498-567) and fsub, fstep (fractional-sample
 // correction, fxkernel.cpp:595-634).  Phases in FP64, reduced, rounded to
-// FP32, then sincosf.
+// FP32, then the hardware sine/cosine (v_sin_f32/v_cos_f32 via __sincosf).
+// Every argument is already reduced: the fringe phases to (-2 pi, 0], the
+// fractional-sample phases to |2 pi c bw/nchan * frac| < 2 pi (|frac| is at
+// most a sample), where the hardware result is within ~5e-7 of libm
+// sincosf (the 2^-24 rounding of x/2 pi times 2 pi, plus the unit's own
+// error); IPP's ippsSinCos_32f_A11, which fxkernel uses, is itself only
+// ~11-bit accurate.
 template <int L, bool CPLX>
 __device__ __forceinline__ void fill_tables(const FParams &p, double frac, double a, double b, float2 *tab,
                                             int e0, int estep) {
@@ -210,20 +216,20 @@ __device__ __forceinline__ void fill_tables(const FParams &p, double frac, doubl
       ph = ph * p.lo;
       if (p.fraclo) ph = ph + p.lofrac * intdelay;
       const double arg = (-2.0 * kPi) * (ph - trunc(ph));
-      sincosf((float)arg, &sn, &cs);
+      __sincosf((float)arg, &sn, &cs);
     } else if (e < S2 + NSTEPS) {
       // stepphase, fxkernel.cpp:112,511,522,539
       const int i = e - S2;
       double ph = ((double)(i * p.stepmul) / (double)L) * a;
       ph = ph * p.lo;
       const double arg = (-2.0 * kPi) * (ph - trunc(ph));
-      sincosf((float)arg, &sn, &cs);
+      __sincosf((float)arg, &sn, &cs);
     } else if (e < S2 + NSTEPS + S) {
       // fxkernel.cpp:600,608
-      sincosf(p.subchanfreq[e - S2 - NSTEPS] * ffrac, &sn, &cs);
+      __sincosf(p.subchanfreq[e - S2 - NSTEPS] * ffrac, &sn, &cs);
     } else {
       // fxkernel.cpp:603,611
-      sincosf(p.stepchanfreq[e - S2 - NSTEPS - S] * ffrac, &sn, &cs);
+      __sincosf(p.stepchanfreq[e - S2 - NSTEPS - S] * ffrac, &sn, &cs);
     }
     tab[e] = make_float2(cs, sn);
   }

@@ -44,8 +44,9 @@ __device__ __attribute__((noinline)) void multi_unit(const FSeg *g, int blk, flo
 template <int OUT, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) fstage_multi_kernel(const FSeg *segs, int nseg, long long total) {
   __shared__ float2 smem[MultiLds<BLOCK>::N];
-  for (long long v = blockIdx.x; v < total; v += gridDim.x) {
-    const long long u = xcd_order(v, total);  // gridDim.x is a multiple of the XCD count
+  // Units in plain order: their cost differs by subband (FFT length), and
+  // striding them over the grid spreads every subband over every XCD.
+  for (long long u = blockIdx.x; u < total; u += gridDim.x) {
     int k = 0;
     while (k + 1 < nseg && u >= segs[k + 1].unit0) ++k;
     const FSeg &g = segs[k];
