@@ -50,14 +50,19 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(workload, precision):
-    """HBM bytes per F-stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), committed as profiles/traffic_<workload>_<precision>.json."""
+def traffic_from_profiles(workload, precision, fblock):
+    """HBM bytes per launch of each stage measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy), committed as
+    profiles/traffic_<workload>_<precision>.json for the F-stage workgroup
+    size the autotuner chose there; None when this run chose another."""
     p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (workload, precision))
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f)
+        t = json.load(f)
+    if t.get("fblock") != fblock:
+        return None
+    return t
 
 
 def cpu_baseline(w, poly, offs, target_s):
@@ -201,7 +206,7 @@ def main():
     }
     dom = "fstage" if f_ms >= x_ms else "xstage"
     achieved = stages[dom]["GBps"]
-    traffic = traffic_from_profiles(args.workload, args.precision)
+    traffic = traffic_from_profiles(args.workload, args.precision, (tuning or {}).get("fblock"))
     roofline = {
         "kernel": dom,
         "bound": "hbm",
