@@ -125,6 +125,11 @@ def load():
         "gxc_fstage": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, vp, u8p]),
         "gxc_xengine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, vp]),
+        "gxc_host_alloc": (vp, [ctypes.c_size_t]),
+        "gxc_host_free": (None, [vp]),
+        "gxc_integrate_begin": (ctypes.c_int, [H]),
+        "gxc_submit": (ctypes.c_int, [H, P(u8p), P(dp), dp]),
+        "gxc_integrate_end": (ctypes.c_int, [H, vp, ctypes.c_size_t, ip]),
         "gxc_set_profiling": (ctypes.c_int, [H, ctypes.c_int]),
         "gxc_autotune": (ctypes.c_int, [H, P(Tuning)]),
         "gxc_get_timings": (ctypes.c_int, [H, P(Timings)]),

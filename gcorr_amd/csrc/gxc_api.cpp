@@ -83,6 +83,22 @@ struct gxc_handle {
   // graph replay
   hipGraphExec_t graph_exec = nullptr;
 
+  // streaming integration (gxc_integrate_begin / gxc_submit / gxc_integrate_end):
+  // two input slots, so the copy of subintegration k+1 (copy stream) runs
+  // while subintegration k is correlated
+  hipStream_t cstream = nullptr;
+  struct Slot {
+    uint8_t *d_in = nullptr;           // [nant][station_bytes]
+    const uint8_t **d_ptrs = nullptr;  // [nant] into d_in
+    double *d_delays = nullptr;        // [nant][4]
+    double *h_delays = nullptr;        // pinned staging of the same
+    hipEvent_t copied = nullptr, done = nullptr;
+    bool used = false;
+  } slot[2];
+  float2 *d_acc = nullptr;  // sum of the subintegrations' visibilities
+  long long nsubmitted = 0;
+  bool integrating = false;
+
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> tev;
@@ -493,6 +509,17 @@ void gxc_destroy(gxc_handle *h) {
   hipEvent_t evs[6] = {h->ev_fork, h->ev_join, h->ev_f[0], h->ev_f[1], h->ev_x[0], h->ev_x[1]};
   for (hipEvent_t e : evs)
     if (e) hipEventDestroy(e);
+  if (h->cstream) hipStreamSynchronize(h->cstream);
+  for (auto &sl : h->slot) {
+    hipFree(sl.d_in);
+    hipFree(sl.d_ptrs);
+    hipFree(sl.d_delays);
+    if (sl.h_delays) hipHostFree(sl.h_delays);
+    if (sl.copied) hipEventDestroy(sl.copied);
+    if (sl.done) hipEventDestroy(sl.done);
+  }
+  hipFree(h->d_acc);
+  if (h->cstream) hipStreamDestroy(h->cstream);
   for (void *p : h->d_spec) hipFree(p);
   hipFree(h->d_partial);
   hipFree(h->d_scales);
@@ -873,6 +900,101 @@ gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out) {
   t.default_ms = def_ms;
   t.candidates = cand;
   if (out) *out = t;
+  return GXC_OK;
+}
+
+void *gxc_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void gxc_host_free(void *p) {
+  if (p) hipHostFree(p);
+}
+
+gxc_status gxc_integrate_begin(gxc_handle *h) {
+  if (!h) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null handle");
+  GXC_HIP(h, hipStreamSynchronize(h->stream));
+  if (!h->cstream) GXC_HIP(h, hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+  const size_t nvis = (size_t)h->d.nrows * 4 * h->cfg.nchan;
+  if (!h->d_acc) GXC_TRY(dmalloc(h, &h->d_acc, nvis));
+  for (auto &sl : h->slot) {
+    if (sl.d_in) continue;
+    const int nant = h->cfg.nant;
+    GXC_TRY(dmalloc(h, &sl.d_in, h->d.station_bytes * (size_t)nant));
+    GXC_TRY(dmalloc(h, &sl.d_ptrs, (size_t)nant));
+    GXC_TRY(dmalloc(h, &sl.d_delays, (size_t)nant * 4));
+    GXC_HIP(h, hipHostMalloc((void **)&sl.h_delays, (size_t)nant * 4 * sizeof(double), hipHostMallocDefault));
+    GXC_HIP(h, hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+    GXC_HIP(h, hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    std::vector<const uint8_t *> ptrs(nant);
+    for (int a = 0; a < nant; a++) ptrs[a] = sl.d_in + (size_t)a * h->d.station_bytes;
+    GXC_HIP(h, hipMemcpy(sl.d_ptrs, ptrs.data(), nant * sizeof(void *), hipMemcpyHostToDevice));
+  }
+  GXC_HIP(h, gxc::launch_fill(reinterpret_cast<float *>(h->d_acc), 2 * nvis, 0.f, h->stream));
+  for (auto &sl : h->slot) sl.used = false;
+  h->nsubmitted = 0;
+  h->integrating = true;
+  return GXC_OK;
+}
+
+gxc_status gxc_submit(gxc_handle *h, const uint8_t *const *per_station, const double *const *poly3,
+                      const double *fileoffsets) {
+  if (!h) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null handle");
+  if (!h->integrating) return set_err(h, GXC_ERR_STATE, "gxc_integrate_begin first");
+  if (!per_station || !poly3 || !fileoffsets) return set_err(h, GXC_ERR_INVALID_ARG, "null input or delays");
+  const int nant = h->cfg.nant;
+  for (int a = 0; a < nant; a++)
+    if (!per_station[a] || !poly3[a]) return set_err(h, GXC_ERR_INVALID_ARG, "null station buffer or polynomial");
+  auto &sl = h->slot[h->nsubmitted & 1];
+  h->marks.clear();
+  // the slot's previous copy (subintegration k-2, already waited on by the
+  // previous call) has finished reading the delay staging before it is rewritten
+  if (sl.used) GXC_HIP(h, hipEventSynchronize(sl.copied));
+  for (int a = 0; a < nant; a++) {
+    for (int i = 0; i < 3; i++) sl.h_delays[4 * a + i] = poly3[a][i];
+    sl.h_delays[4 * a + 3] = fileoffsets[a];
+  }
+  // copy stream: after subintegration k-2 released the slot
+  if (sl.used) GXC_HIP(h, hipStreamWaitEvent(h->cstream, sl.done, 0));
+  for (int a = 0; a < nant; a++)
+    GXC_HIP(h, hipMemcpyAsync(sl.d_in + (size_t)a * h->d.station_bytes, per_station[a], h->d.station_bytes,
+                              hipMemcpyHostToDevice, h->cstream));
+  GXC_HIP(h, hipMemcpyAsync(sl.d_delays, sl.h_delays, (size_t)nant * 4 * sizeof(double), hipMemcpyHostToDevice,
+                            h->cstream));
+  GXC_HIP(h, hipEventRecord(sl.copied, h->cstream));
+  // compute stream: correlate from the slot, then add into the sum
+  GXC_HIP(h, hipStreamWaitEvent(h->stream, sl.copied, 0));
+  const uint8_t **ptrs = h->d_station_ptrs;
+  double *dl = h->d_delays;
+  h->d_station_ptrs = sl.d_ptrs;
+  h->d_delays = sl.d_delays;
+  gxc_status st = enqueue_device_work(h);
+  h->d_station_ptrs = ptrs;
+  h->d_delays = dl;
+  GXC_TRY(st);
+  const size_t nvis = (size_t)h->d.nrows * 4 * h->cfg.nchan;
+  GXC_HIP(h, gxc::launch_add(reinterpret_cast<float *>(h->d_acc), reinterpret_cast<const float *>(h->d_vis),
+                             2 * nvis, h->stream));
+  GXC_HIP(h, hipEventRecord(sl.done, h->stream));
+  sl.used = true;
+  // the previous subintegration's host buffers have been read once this
+  // returns, so a caller may refill them while this one copies
+  if (h->nsubmitted > 0) GXC_HIP(h, hipEventSynchronize(h->slot[(h->nsubmitted - 1) & 1].copied));
+  h->nsubmitted++;
+  return GXC_OK;
+}
+
+gxc_status gxc_integrate_end(gxc_handle *h, gxc_cf32 *out, size_t nelem, int *nsubints) {
+  if (!h) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null handle");
+  if (!h->integrating) return set_err(h, GXC_ERR_STATE, "no integration in progress");
+  const size_t nvis = (size_t)h->d.nrows * 4 * h->cfg.nchan;
+  if (out && nelem < nvis) return set_err(h, GXC_ERR_INVALID_ARG, "output buffer too small");
+  GXC_HIP(h, hipStreamSynchronize(h->stream));
+  if (out) GXC_HIP(h, hipMemcpy(out, h->d_acc, nvis * sizeof(gxc_cf32), hipMemcpyDeviceToHost));
+  if (nsubints) *nsubints = (int)h->nsubmitted;
+  h->integrating = false;
   return GXC_OK;
 }
 

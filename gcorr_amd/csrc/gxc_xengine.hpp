@@ -60,6 +60,8 @@ int xm_tiles(int M, int out_mode, int *out);
 hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s);
 constexpr int kPowerRows = 64;  // FFTs sampled for the scale estimate
 hipError_t launch_finalise_mixed(const FinMParams &p, hipStream_t s);
+// acc[i] += v[i] for n floats.
+hipError_t launch_add(float *acc, const float *v, size_t n, hipStream_t s);
 // Fill n floats with v (a kernel, so it may sit inside a captured graph).
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 // Pack FP32 complex spectra [n] with unit scale into the out_mode format.

@@ -397,6 +397,11 @@ __global__ void __launch_bounds__(256) finalise_multi_kernel(const FinSeg *segs,
   }
 }
 
+__global__ void add_kernel(float *acc, const float *v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) acc[i] += v[i];
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -640,6 +645,12 @@ hipError_t launch_scales_multi(const ScaleSeg *segs, int nseg, long long total, 
 
 hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s) {
   hipLaunchKernelGGL(scales_kernel, dim3((n + 255) / 256), dim3(256), 0, s, scales, power, n, nrows, target);
+  return hipGetLastError();
+}
+
+hipError_t launch_add(float *acc, const float *v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc, v, n);
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@ by tests and benchmarks.  Visibilities are [row][product][channel] complex64:
 cross baselines in fxkernel order (fxkernel.cpp:328-355), product p = 2l+m
 = X_j,l * conj(X_k,m), then (with autos=True) one row per station.
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -43,6 +44,34 @@ def num_rows(nant, autos=False):
     return nant * (nant - 1) // 2 + (nant if autos else 0)
 
 
+class PinnedBuffer:
+    """Page-locked host memory from gxc_host_alloc, viewed as a uint8 array.
+
+    Station buffers in pinned memory let gxc_submit copy them to the device
+    asynchronously; pageable buffers are copied synchronously."""
+
+    def __init__(self, nbytes):
+        self._L = _lib.load()
+        self.nbytes = int(nbytes)
+        self._p = self._L.gxc_host_alloc(self.nbytes)
+        if not self._p:
+            raise MemoryError("gxc_host_alloc(%d) failed" % self.nbytes)
+        self.array = np.ctypeslib.as_array(ctypes.cast(self._p, ctypes.POINTER(ctypes.c_uint8)),
+                                           shape=(self.nbytes,))
+
+    def free(self):
+        if getattr(self, "_p", None):
+            self.array = None
+            self._L.gxc_host_free(self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class FxKernel:
     """FxKernel(nant, nchan, nfft, nbit, lo, bw, is_complex) on the MI355X path."""
 
@@ -59,6 +88,7 @@ class FxKernel:
         check(self._L.gxc_create(ctypes.byref(self.cfg), ctypes.byref(h)))
         self._h = h
         self._keep = None
+        self._inflight = collections.deque(maxlen=2)
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -158,6 +188,44 @@ class FxKernel:
         check(self._L.gxc_fstage(self._h, int(t0), int(nfft), spec.ctypes.data_as(ctypes.c_void_p),
                                  valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), self._h)
         return spec, valid.astype(bool)
+
+    def integrate_begin(self):
+        """Start a streamed sum of subintegrations (gxc_integrate_begin)."""
+        check(self._L.gxc_integrate_begin(self._h), self._h)
+        self._inflight.clear()
+
+    def submit(self, idata, d, f):
+        """Queue one subintegration from host station buffers (numpy uint8 or
+        PinnedBuffer arrays) with its delays; its visibilities are added to
+        the sum.  Returns before the work is done; the previous submission's
+        buffers may be refilled once this returns.  Buffers are kept alive
+        here until they have been read."""
+        arrs = [a if isinstance(a, np.ndarray) and a.dtype == np.uint8 and a.flags.c_contiguous
+                else np.ascontiguousarray(a, dtype=np.uint8) for a in idata]
+        if len(arrs) != self.nant:
+            raise ValueError("need %d station buffers" % self.nant)
+        for a in arrs:
+            if a.size < self.station_bytes:
+                raise ValueError("station buffer has %d bytes, need %d" % (a.size, self.station_bytes))
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(self.nant, 3)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.nant)
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * self.nant)(
+            *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) for a in arrs])
+        rows = (ctypes.POINTER(ctypes.c_double) * self.nant)(
+            *[d[a].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in range(self.nant)])
+        check(self._L.gxc_submit(self._h, ptrs, rows, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              self._h)
+        self._inflight.append(arrs)
+
+    def integrate_end(self):
+        """Wait for the submitted subintegrations; returns (sum, count), the
+        sum a complex64 array [rows][4][nchan]."""
+        out = np.empty((self.nrows, 4, self.nchan), np.complex64)
+        n = ctypes.c_int(0)
+        check(self._L.gxc_integrate_end(self._h, out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                        ctypes.byref(n)), self._h)
+        self._inflight.clear()
+        return out, n.value
 
     def autotune(self):
         """gxc_autotune: time the launch shapes on this handle's input and keep

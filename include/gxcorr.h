@@ -206,6 +206,30 @@ typedef struct {
 } gxc_tuning;
 gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out);
 
+/* ---- Streaming subintegrations from host memory ----------------------------
+ * fxkernel processes one subintegration per process() and the caller sums
+ * them with accumulate() (fxkernel.cpp:414-423); gcorr's driver overlaps host
+ * copies with compute on two streams (testgpukernel.cu:448-486).  These calls
+ * pipeline that on one handle: gxc_submit queues the copy of a
+ * subintegration's input (host, one buffer per station of gxc_station_bytes)
+ * and delays into one of two device slots on a copy stream, then its
+ * correlation on the handle's stream, and adds its visibilities to a device
+ * sum; the copy of subintegration k+1 runs while subintegration k is
+ * correlated.  Host buffers from gxc_host_alloc (pinned) copy asynchronously;
+ * any other host memory works but copies synchronously.  A submission's
+ * station buffers may be refilled once the next gxc_submit (or
+ * gxc_integrate_end) has returned, so two sets of host buffers suffice;
+ * delays are copied at the call.  The sum is
+ * Σ_k vis_k in submission order, each vis_k normalised as gxc_process does. */
+void *gxc_host_alloc(size_t bytes);  /* pinned host memory, NULL on failure */
+void gxc_host_free(void *p);
+gxc_status gxc_integrate_begin(gxc_handle *h);  /* zero the sum */
+gxc_status gxc_submit(gxc_handle *h, const uint8_t *const *per_station,
+                      const double *const *poly3, const double *fileoffsets);
+/* Wait for every submitted subintegration; the sum [rows][4][nchan] goes to
+ * out (may be NULL), the number summed to *nsubints (may be NULL). */
+gxc_status gxc_integrate_end(gxc_handle *h, gxc_cf32 *out, size_t nelem, int *nsubints);
+
 /* Per-stage timing (see gxc_timings).  Enabled by gxc_set_profiling. */
 gxc_status gxc_set_profiling(gxc_handle *h, int enable);
 gxc_status gxc_get_timings(const gxc_handle *h, gxc_timings *t);

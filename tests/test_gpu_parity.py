@@ -118,7 +118,8 @@ def test_process_c2_shape_reduced(gpu):
 
 
 @pytest.mark.parametrize("nant,nchan,numffts", [(1, 256, 50), (2, 64, 300), (5, 512, 97),
-                                                (9, 128, 64), (16, 256, 24)])
+                                                (9, 128, 64), (16, 256, 24), (3, 1024, 1),
+                                                (4, 32, 2), (32, 256, 12)])
 def test_process_shapes(gpu, nant, nchan, numffts):
     w, data, poly, offs = random_case(nant, nchan, numffts, seed=nant * 100 + nchan)
     v, c = run_gpu(w, data, poly, offs)
@@ -258,11 +259,17 @@ def test_accumulate_and_save(gpu, tmp_path):
 
 
 def test_errors_are_reported(gpu):
+    import torch
     k = gcorr_amd.FxKernel(2, 64, 10, 2, 1.65e9, 16e6)
     with pytest.raises(gcorr_amd.GxcError):
         k.process()                    # no input, no delays
     with pytest.raises(ValueError):
         k.setInputData([np.zeros(10, np.uint8)] * 2)
+    buf = torch.zeros(k.station_bytes + 8, dtype=torch.uint8, device="cuda:0")
+    with pytest.raises(gcorr_amd.GxcError):  # device buffers must be 4-byte aligned
+        k.setInputDevice([buf.data_ptr() + 1, buf.data_ptr()], keepalive=buf)
+    with pytest.raises(gcorr_amd.GxcError):  # not an F-stage workgroup size
+        gcorr_amd.FxKernel(2, 64, 10, 2, 1.65e9, 16e6, fblock=96)
     k.close()
 
 
