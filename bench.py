@@ -40,6 +40,7 @@ def parse():
                     help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-h2d", action="store_true")
+    ap.add_argument("--no-vdif", action="store_true", help="skip the VDIF ingest measurement")
     ap.add_argument("--overlap", action="store_true",
                     help="F of chunk k+1 on a second stream beside X of chunk k")
     ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
@@ -260,6 +261,51 @@ def main():
         kh.close()
         del host
 
+    # VDIF ingest (gxc_set_input_vdif): the same input as 8000-byte-payload
+    # frames in HBM, placed by the GPU, then correlated; wall-clock per
+    # subintegration, every station ingested once.
+    vd = None
+    if not args.no_vdif and args.workload != "c4" and w.nant * w.station_bytes <= (4 << 30):
+        from gcorr_amd import vdif
+        payload = 8000
+        spf = payload * 8 // (w.nbit * 2 * (2 if w.is_complex else 1))
+        fps = max(1, int(round(w.fftchannels / spf * w.numffts)))
+        frames = [torch.from_numpy(vdif.frames_from_station(b.cpu().numpy(), payload, w.nbit, w.is_complex,
+                                                            frames_per_second=fps).reshape(-1)).to(dev)
+                  for b in bufs]
+        shape = {} if tuning is None else {key: tuning[key] for key in ("fblock", "xtile", "xsplits")}
+        kv = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
+                                precision=args.precision, autos=True, **shape)
+        kv.setDelays(poly, offs)
+
+        def ingest():
+            for a, f in enumerate(frames):
+                kv.setInputVDIF(a, f.data_ptr(), f.numel(), frames_per_second=fps,
+                                frame_bytes=payload + 32, stats=False)
+
+        ingest()
+        kv.process()
+        same = bool(np.array_equal(kv.visibilities(), k.visibilities()))
+        n = 5
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            ingest()
+        kv.synchronize()
+        ing = (time.perf_counter() - t) / n
+        t = time.perf_counter()
+        for _ in range(n):
+            ingest()
+            kv.process()
+        both = (time.perf_counter() - t) / n
+        fbytes = sum(f.numel() for f in frames)
+        vd = {"payload_bytes": payload, "frame_bytes": payload + 32, "ingest_ms": ing * 1e3,
+              "ingest_GBps": (fbytes + w.nant * w.station_bytes) / ing / 1e9,
+              "with_process_gsamples_per_s": input_samples / both / 1e9,
+              "visibilities_equal_device_input": same}
+        kv.close()
+        del frames
+
     cpu = None
     if not args.no_cpu_baseline and args.workload != "c4":
         cpu = cpu_baseline(w, poly, offs, args.cpu_seconds)
@@ -294,6 +340,7 @@ def main():
         "tuning": tuning,
         "gbps_input": value * (16 if w.is_complex else 2),
         "pcie_inclusive": pcie,
+        "vdif": vd,
         "cpu_baseline": cpu,
     }
     k.close()

@@ -45,6 +45,21 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class VdifSpec(ctypes.Structure):
+    _fields_ = [
+        ("frame_bytes", ctypes.c_int), ("thread_id", ctypes.c_int),
+        ("frames_per_second", ctypes.c_int), ("ref_epoch", ctypes.c_int),
+        ("ref_second", ctypes.c_longlong), ("ref_frame", ctypes.c_int),
+        ("offset_binary8", ctypes.c_int), ("reserved", ctypes.c_int * 2),
+    ]
+
+
+class VdifStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in
+                ("frames", "placed", "missing", "duplicate", "invalid", "other_thread",
+                 "out_of_window", "malformed")]
+
+
 class Timings(ctypes.Structure):
     _fields_ = [
         ("h2d_ms", ctypes.c_double), ("fstage_ms", ctypes.c_double),
@@ -112,6 +127,12 @@ def load():
         "gxc_set_input": (ctypes.c_int, [H, P(u8p)]),
         "gxc_set_input_device": (ctypes.c_int, [H, P(vp)]),
         "gxc_set_delays": (ctypes.c_int, [H, P(dp), dp]),
+        "gxc_vdif_spec_init": (None, [P(VdifSpec)]),
+        "gxc_vdif_slots": (ctypes.c_longlong, [P(Config), ctypes.c_int, ctypes.c_int]),
+        "gxc_set_input_vdif": (ctypes.c_int, [H, ctypes.c_int, vp, ctypes.c_size_t, P(VdifSpec),
+                                              P(VdifStats)]),
+        "gxc_get_frame_flags": (ctypes.c_int, [H, ctypes.c_int, u8p, ctypes.c_size_t,
+                                               P(ctypes.c_longlong)]),
         "gxc_process": (ctypes.c_int, [H]),
         "gxc_process_async": (ctypes.c_int, [H]),
         "gxc_synchronize": (ctypes.c_int, [H]),

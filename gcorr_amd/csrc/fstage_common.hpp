@@ -184,7 +184,17 @@ __device__ __forceinline__ bool split_offset(const FParams &p, int st, int t, in
     frac -= p.sampletime;
   }
   offset = (int)off;
-  return !(off < 0 || off > p.maxoffset);
+  if (off < 0 || off > p.maxoffset) return false;
+  if (p.frame_ok) {
+    // samples [off, off + L) lie in frames off/spf .. (off+L-1)/spf, all
+    // below the station's slot count since off + L <= (numffts+1) L
+    const uint8_t *ok = p.frame_ok[st];
+    const long long spf = p.frame_samples[st];
+    const long long f1 = (off + L - 1) / spf;
+    for (long long f = off / spf; f <= f1; f++)
+      if (!ok[f]) return false;
+  }
+  return true;
 }
 
 // Entries e = e0, e0+estep, ... of one item's rotator tables: sub, step

@@ -48,6 +48,11 @@ struct FParams {
   const float *scales;            // [nant*2][nchan] multipliers before rounding
   float *power;                   // OUT_POWER: [nt][nant*2][nchan] |X|^2 per sampled FFT
   uint8_t *valid;                 // [nant][numffts]
+  // Missing VDIF frames (nullptr: none): frame_ok[st][f] != 0 when frame f,
+  // samples [f, f+1) * frame_samples[st], is present; an FFT touching a
+  // missing frame is invalid for that station.
+  const uint8_t *const *frame_ok;
+  const int *frame_samples;
   int out_mode;
   int t0, nt, tstride, nant, numffts;
   int maxoffset;                  // (numffts-1)*L

@@ -89,6 +89,7 @@ class FxKernel:
         self._h = h
         self._keep = None
         self._inflight = collections.deque(maxlen=2)
+        self._vdif_keep = {}
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -129,6 +130,36 @@ class FxKernel:
         ptrs = (ctypes.c_void_p * self.nant)(*[int(p) for p in dev_ptrs])
         check(self._L.gxc_set_input_device(self._h, ptrs), self._h)
         self._keep = (keepalive, ptrs)
+
+    def setInputVDIF(self, station, dev_ptr, nbytes, *, frames_per_second, ref_second=0,
+                     ref_frame=0, ref_epoch=0, thread_id=0, frame_bytes=0, offset_binary8=True,
+                     keepalive=None, stats=True):
+        """Station `station`'s input from VDIF frames in device memory
+        (gxc_set_input_vdif).  Returns the ingest counts as a dict (stats=True
+        waits for the ingest)."""
+        spec = _lib.VdifSpec()
+        self._L.gxc_vdif_spec_init(ctypes.byref(spec))
+        spec.frame_bytes, spec.thread_id = int(frame_bytes), int(thread_id)
+        spec.frames_per_second, spec.ref_epoch = int(frames_per_second), int(ref_epoch)
+        spec.ref_second, spec.ref_frame = int(ref_second), int(ref_frame)
+        spec.offset_binary8 = int(bool(offset_binary8))
+        st = _lib.VdifStats()
+        check(self._L.gxc_set_input_vdif(self._h, int(station), ctypes.c_void_p(int(dev_ptr)),
+                                         int(nbytes), ctypes.byref(spec),
+                                         ctypes.byref(st) if stats else None), self._h)
+        # the ingest is queued: the frames must live until it has run
+        self._vdif_keep[int(station)] = keepalive
+        return {k: getattr(st, k) for k, _ in _lib.VdifStats._fields_} if stats else None
+
+    def frame_flags(self, station):
+        """Per-slot presence flags of the station's last VDIF ingest."""
+        n = ctypes.c_longlong(0)
+        check(self._L.gxc_get_frame_flags(self._h, int(station), None, 0, ctypes.byref(n)), self._h)
+        out = np.empty(n.value, np.uint8)
+        check(self._L.gxc_get_frame_flags(self._h, int(station),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), out.size,
+                                          None), self._h)
+        return out
 
     def setDelays(self, d, f):
         """d: [nant][3] polynomial (s, t in FFTs), f: [nant] file offsets (s)."""
@@ -245,6 +276,7 @@ class FxKernel:
     # snake_case aliases
     set_input_data = setInputData
     set_input_device = setInputDevice
+    set_input_vdif = setInputVDIF
     set_delays = setDelays
     save_visibilities = saveVisibilities
 

@@ -134,6 +134,71 @@ gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station);
  * gcorr/src/testgpukernel.cu:468-470). */
 gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station);
 
+/* ---- VDIF ingest --------------------------------------------------------
+ * gcorr's ingest (ingest/src/ingest.c:267-355) reads VDIF frames, strips
+ * the headers and places each payload by its time stamp:
+ *   slot = (seconds - ref_second) * frames_per_second + frame - ref_frame
+ * (ingest.c:315).  gxc_set_input_vdif does that on the GPU for one
+ * station of the next subintegration: frames already in device memory,
+ * fixed frame size, two channels (the polarisations) per frame, nbit and
+ * real/complex as configured.  Header fields follow the VDIF layout that
+ * examples/mark/dump_vdif.py:36-55 decodes.  The first frame of the stream
+ * for each slot is kept; a frame is skipped when its thread differs from
+ * thread_id (>= 0), its format differs from the configuration, its invalid
+ * bit is set, or it falls outside the subintegration.  Slots that receive
+ * no frame are zero-filled and flagged missing, and every FFT whose input
+ * samples touch a missing frame is invalid for that station (it is left
+ * out of the sums and the row counts, as fxkernel does with delays out of
+ * range).  8-bit samples are offset binary in VDIF and are converted to
+ * two's complement (offset_binary8 = 1). */
+typedef struct {
+  int frame_bytes;        /* header + payload bytes of every frame; 0: from the
+                             first frame's length field */
+  int thread_id;          /* VDIF thread to keep; -1 keeps every thread */
+  int frames_per_second;  /* frames per second of the kept thread */
+  int ref_epoch;          /* reference epoch (6-bit VDIF field) of sample 0 */
+  long long ref_second;   /* seconds from the epoch of sample 0 */
+  int ref_frame;          /* frame number within that second of sample 0 */
+  int offset_binary8;     /* 8-bit samples: 1 = offset binary (VDIF), 0 = two's complement */
+  int reserved[2];
+} gxc_vdif_spec;
+
+typedef struct {
+  long long frames;        /* whole frames in the stream */
+  long long placed;        /* slots filled */
+  long long missing;       /* slots with no frame (flagged) */
+  long long duplicate;     /* frames for an already filled slot */
+  long long invalid;       /* frames with the invalid bit set */
+  long long other_thread;  /* frames of another thread */
+  long long out_of_window; /* frames outside the subintegration or another epoch */
+  long long malformed;     /* frames whose length, channel count, bits or
+                              data type differ from the configuration */
+} gxc_vdif_stats;
+
+void gxc_vdif_spec_init(gxc_vdif_spec *spec); /* thread 0, offset binary, rest 0 */
+
+/* Station `station`'s input for the next gxc_process from `nbytes` of VDIF
+ * frames at device address d_frames (8-byte aligned).  The ingest is queued
+ * on the handle's stream and copies the payloads into a buffer the handle
+ * owns: d_frames may be reused once gxc_process, gxc_synchronize or a call
+ * with stats non-NULL has returned.  With spec->frame_bytes set the call
+ * does not wait for the device; with 0 it reads the first header first.
+ * Every station must be ingested this way before gxc_process;
+ * gxc_set_input / gxc_set_input_device switch the handle back to
+ * unflagged input.  With stats non-NULL the call waits for the ingest and
+ * reports its counts. */
+gxc_status gxc_set_input_vdif(gxc_handle *h, int station, const void *d_frames, size_t nbytes,
+                              const gxc_vdif_spec *spec, gxc_vdif_stats *stats);
+
+/* Frames per station buffer for a spec's frame size (slots of one
+ * subintegration), or 0 if frame_bytes does not hold whole samples. */
+long long gxc_vdif_slots(const gxc_config *cfg, int frame_bytes, int legacy_header);
+
+/* The missing-frame flags of station `station` from the last ingest:
+ * out[f] = 1 when slot f holds a frame.  *nslots (may be NULL) gets the
+ * slot count; out may be NULL to ask for the count only. */
+gxc_status gxc_get_frame_flags(gxc_handle *h, int station, uint8_t *out, size_t n, long long *nslots);
+
 /* FxKernel::setDelays (fxkernel/src/fxkernel.cpp:247-251): poly3[a] points
  * to 3 doubles, fileoffsets[a] seconds.  Values are copied at this call. */
 gxc_status gxc_set_delays(gxc_handle *h, const double *const *poly3,

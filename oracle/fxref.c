@@ -95,6 +95,84 @@ int fxref_delay_split(const fxref_config *c, const double poly[3],
   return !(off < 0 || off > d.maxoffset);
 }
 
+/* Missing-frame rule (an extension, see fxref.h): station st's FFT input
+ * samples [offset, offset + L) must lie in frames flagged present. */
+static int frames_present(const fxref_config *c, int st, long long offset, int L) {
+  if (!c->frame_ok || st < 0) return 1;
+  const uint8_t *ok = c->frame_ok[st];
+  const long long spf = c->frame_samples[st];
+  for (long long f = offset / spf; f <= (offset + L - 1) / spf; f++)
+    if (!ok[f]) return 0;
+  return 1;
+}
+
+static uint32_t le32(const uint8_t *p) {
+  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+}
+
+long long fxref_vdif_place(const fxref_config *c, const uint8_t *stream,
+                           size_t nbytes, const fxref_vdif_spec *spec,
+                           uint8_t *station, uint8_t *frame_ok,
+                           int *frame_samples, fxref_vdif_stats *stats) {
+  fxref_derived d;
+  if (fxref_derive(c, &d) || nbytes < 16 || spec->frames_per_second <= 0) return -1;
+  memset(stats, 0, sizeof(*stats));
+  const uint32_t legacy0 = (le32(stream) >> 30) & 1;
+  const long long hdr = legacy0 ? 16 : 32;
+  const long long fb = spec->frame_bytes ? spec->frame_bytes
+                                         : 8LL * (le32(stream + 8) & 0xffffff);
+  const long long payload = fb - hdr;
+  const long long bytes_per_time = (long long)c->nbit * 2 * (c->is_complex ? 2 : 1); /* bits */
+  if (payload <= 0 || payload % 8 || (payload * 8) % bytes_per_time) return -1;
+  *frame_samples = (int)(payload * 8 / bytes_per_time);
+  const long long sb = (long long)d.bytes_per_station;
+  const long long nslots = (sb + payload - 1) / payload;
+  const long long nframes = (long long)(nbytes / (size_t)fb); /* whole frames only, ingest.c:303 */
+  stats->frames = nframes;
+  long long *src = (long long *)malloc(sizeof(long long) * (size_t)nslots);
+  if (!src) return -1;
+  for (long long s = 0; s < nslots; s++) src[s] = -1;
+  for (long long i = 0; i < nframes; i++) {
+    const uint8_t *f = stream + i * fb;
+    const uint32_t w0 = le32(f), w1 = le32(f + 4), w2 = le32(f + 8), w3 = le32(f + 12);
+    const int thread = (int)((w3 >> 16) & 0x3ff);
+    if (spec->thread_id >= 0 && thread != spec->thread_id) { stats->other_thread++; continue; }
+    const long long frame = w1 & 0xffffff;
+    if (8LL * (w2 & 0xffffff) != fb || ((w0 >> 30) & 1) != legacy0 || ((w2 >> 24) & 0x1f) != 1 ||
+        (int)((w3 >> 26) & 0x1f) + 1 != c->nbit || (int)(w3 >> 31) != (c->is_complex ? 1 : 0) ||
+        frame >= spec->frames_per_second) {
+      stats->malformed++;
+      continue;
+    }
+    if (w0 >> 31) { stats->invalid++; continue; }
+    const long long seconds = w0 & 0x3fffffff;
+    const long long rel = (seconds - spec->ref_second) * spec->frames_per_second + frame - spec->ref_frame;
+    if ((int)((w1 >> 24) & 0x3f) != spec->ref_epoch || rel < 0 || rel >= nslots) {
+      stats->out_of_window++;
+      continue;
+    }
+    if (src[rel] >= 0) { stats->duplicate++; continue; }
+    src[rel] = i;
+  }
+  const int flip = c->nbit == 8 && spec->offset_binary8;
+  for (long long s = 0; s < nslots; s++) {
+    const long long n = s * payload + payload <= sb ? payload : sb - s * payload;
+    uint8_t *dst = station + s * payload;
+    if (src[s] >= 0) {
+      const uint8_t *p = stream + src[s] * fb + hdr;
+      for (long long b = 0; b < n; b++) dst[b] = flip ? (uint8_t)(p[b] ^ 0x80) : p[b];
+      frame_ok[s] = 1;
+      stats->placed++;
+    } else {
+      memset(dst, 0, (size_t)n);
+      frame_ok[s] = 0;
+      stats->missing++;
+    }
+  }
+  free(src);
+  return nslots;
+}
+
 #define REAL float
 #define SUF f32
 #define SINCOS_ARG(x, s, co) sincosf((float)(x), (s), (co))

@@ -26,6 +26,11 @@
  *                stride law (fxkernel.cpp:105-114,553-558).
  *   nchan that is not a perfect square: stride s = 2^floor(log2(nchan)/2),
  *                nchan/s steps (fxkernel.cpp:58-63 refuses such nchan).
+ *   frame_ok     missing-frame flags from a VDIF ingest: an FFT whose input
+ *                samples touch a missing frame is invalid for its station
+ *                (fxkernel has no missing data; gcorr's ingest,
+ *                ingest/src/ingest.c:267-355, places frames but flags
+ *                nothing).
  */
 #ifndef FXREF_H
 #define FXREF_H
@@ -47,6 +52,11 @@ typedef struct {
   double bw;      /* bandwidth, Hz */
   int autos;      /* 1: append autocorrelations after the cross baselines */
   int quirk;      /* 1: fxkernel stride phase law (default), 0: exact law */
+  /* Missing-frame flags (NULL: none): frame_ok[station][f] != 0 when frame
+   * f, samples [f*frame_samples[station], (f+1)*frame_samples[station]),
+   * is present. */
+  const uint8_t *const *frame_ok;
+  const int *frame_samples;
 } fxref_config;
 
 typedef struct {
@@ -135,6 +145,44 @@ int fxref_accumulate_range_f32(const fxref_config *c,
  * integer known-answer test of gcorr/test/unit.cc:42-159. */
 void fxref_xmac_f32(const float *spectra, int nant, int nchan, int nfft,
                     int autos, float *vis);
+
+/* ---- VDIF placement (gcorr ingest, ingest/src/ingest.c:267-355) ----------
+ * Header fields as examples/mark/dump_vdif.py:36-55 decodes them (VDIF
+ * words little-endian): w0 = invalid:1 legacy:1 seconds:30; w1 = epoch:6
+ * (bits 24-29) frame:24; w2 = log2(channels):5 (bits 24-28) length/8:24;
+ * w3 = complex:1 (bit 31) bits-1:5 (26-30) thread:10 (16-25) station:16.
+ * Slot of a frame: (seconds - ref_second) * frames_per_second + frame -
+ * ref_frame (ingest.c:315).  Frames are read at a fixed stride of
+ * frame_bytes; checks in order: thread (other_thread), format (malformed:
+ * length, legacy bit differing from the first frame, channels != 2, bits,
+ * data type, frame >= frames_per_second), invalid bit (invalid), epoch and
+ * slot range (out_of_window), slot already taken by an earlier frame
+ * (duplicate).  Slots with a frame get its payload (8-bit offset-binary
+ * samples XOR 0x80 when offset_binary8), the others zeros; frame_ok[slot]
+ * tells which. */
+typedef struct {
+  int frame_bytes; /* 0: from the first frame's length field */
+  int thread_id;   /* -1: every thread */
+  int frames_per_second;
+  int ref_epoch;
+  long long ref_second;
+  int ref_frame;
+  int offset_binary8;
+} fxref_vdif_spec;
+
+typedef struct {
+  long long frames, placed, missing, duplicate, invalid, other_thread,
+      out_of_window, malformed;
+} fxref_vdif_stats;
+
+/* Places the frames of `stream` into one station buffer of the config's
+ * bytes_per_station.  frame_ok has room for the returned slot count (at
+ * most bytes_per_station / 8 + 1).  Returns the slot count and the time
+ * samples per frame in *frame_samples, or -1 for an unusable stream. */
+long long fxref_vdif_place(const fxref_config *c, const uint8_t *stream,
+                           size_t nbytes, const fxref_vdif_spec *spec,
+                           uint8_t *station, uint8_t *frame_ok,
+                           int *frame_samples, fxref_vdif_stats *stats);
 
 /* bench_fxkernel analogue (fxkernel/src/bench_fxkernel.cpp:93-139,238-285):
  * nthreads threads, each with its own private copy of the input and its own

@@ -27,7 +27,27 @@ class Config(ctypes.Structure):
         ("bw", ctypes.c_double),
         ("autos", ctypes.c_int),
         ("quirk", ctypes.c_int),
+        ("frame_ok", ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))),
+        ("frame_samples", ctypes.POINTER(ctypes.c_int)),
     ]
+
+
+class VdifSpec(ctypes.Structure):
+    _fields_ = [
+        ("frame_bytes", ctypes.c_int),
+        ("thread_id", ctypes.c_int),
+        ("frames_per_second", ctypes.c_int),
+        ("ref_epoch", ctypes.c_int),
+        ("ref_second", ctypes.c_longlong),
+        ("ref_frame", ctypes.c_int),
+        ("offset_binary8", ctypes.c_int),
+    ]
+
+
+class VdifStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in
+                ("frames", "placed", "missing", "duplicate", "invalid", "other_thread",
+                 "out_of_window", "malformed")]
 
 
 class Derived(ctypes.Structure):
@@ -94,9 +114,19 @@ def _ptr(a, ct):
 
 
 def make_config(nant, nchan, numffts, lo, bw, nbit=2, is_complex=False, autos=False,
-                quirk=True):
-    return Config(int(nant), int(nchan), int(numffts), int(nbit), int(bool(is_complex)),
-                  float(lo), float(bw), int(bool(autos)), int(bool(quirk)))
+                quirk=True, frame_ok=None, frame_samples=None):
+    """frame_ok: per-station uint8 flags of a VDIF ingest, with
+    frame_samples[station] time samples per frame (None: no missing data)."""
+    c = Config(int(nant), int(nchan), int(numffts), int(nbit), int(bool(is_complex)),
+               float(lo), float(bw), int(bool(autos)), int(bool(quirk)))
+    if frame_ok is not None:
+        flags = [np.ascontiguousarray(f, np.uint8) for f in frame_ok]
+        spf = np.ascontiguousarray(frame_samples, np.int32).reshape(int(nant))
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * int(nant))(*[_ptr(f, ctypes.c_uint8) for f in flags])
+        c.frame_ok = ctypes.cast(ptrs, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)))
+        c.frame_samples = _ptr(spf, ctypes.c_int)
+        c._keep = (flags, spf, ptrs)
+    return c
 
 
 def derive(cfg):
@@ -208,6 +238,33 @@ def process(cfg, data, delays, offsets, mode="f32"):
     if rc:
         raise RuntimeError("oracle process failed (%d)" % rc)
     return vis[..., 0] + 1j * vis[..., 1], counts
+
+
+def vdif_place(cfg, stream, frame_bytes=0, thread_id=0, frames_per_second=1, ref_epoch=0,
+               ref_second=0, ref_frame=0, offset_binary8=True):
+    """Place a VDIF byte stream into one station buffer (gcorr ingest,
+    ingest/src/ingest.c:267-355).  Returns (station bytes, frame_ok flags,
+    samples per frame, stats dict)."""
+    d = derive(cfg)
+    stream = np.ascontiguousarray(stream, np.uint8)
+    spec = VdifSpec(int(frame_bytes), int(thread_id), int(frames_per_second), int(ref_epoch),
+                    int(ref_second), int(ref_frame), int(bool(offset_binary8)))
+    station = np.empty(d.bytes_per_station, np.uint8)
+    flags = np.empty(d.bytes_per_station // 8 + 1, np.uint8)
+    spf = ctypes.c_int(0)
+    st = VdifStats()
+    L = lib()
+    L.fxref_vdif_place.restype = ctypes.c_longlong
+    L.fxref_vdif_place.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_uint8),
+                                   ctypes.c_size_t, ctypes.POINTER(VdifSpec),
+                                   ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(VdifStats)]
+    n = L.fxref_vdif_place(ctypes.byref(cfg), _ptr(stream, ctypes.c_uint8), stream.size,
+                           ctypes.byref(spec), _ptr(station, ctypes.c_uint8),
+                           _ptr(flags, ctypes.c_uint8), ctypes.byref(spf), ctypes.byref(st))
+    if n < 0:
+        raise ValueError("unusable VDIF stream")
+    return station, flags[:n].copy(), spf.value, {k: getattr(st, k) for k, _ in VdifStats._fields_}
 
 
 def accumulate_range(cfg, data, delays, offsets, t0, t1):

@@ -268,12 +268,13 @@ static void FN(ws_free)(FN(ws) * w) {
  * Leaves nchan channelised bins of both pols in w->chan[j]. */
 static int FN(station)(FN(ws) * w, const fxref_config *c, const uint8_t *data,
                        const double poly[3], double fileoffset, int i,
-                       REAL *chan_j) {
+                       REAL *chan_j, int st) {
   const size_t L = (size_t)w->d.fftchannels;
   int offset;
   double frac, a, b;
   if (!fxref_delay_split(c, poly, fileoffset, i, &offset, &frac, &a, &b))
     return 0;
+  if (!frames_present(c, st, offset, (int)L)) return 0; /* missing VDIF frames */
   FN(fxref_unpack)(c, data, offset, w->unpacked);      /* :310 */
   FN(fxref_rotator)(c, a, b, w->rot);                  /* :313 */
   for (int p = 0; p < 2; p++) {                        /* :561-566 */
@@ -305,7 +306,7 @@ int FN(fxref_station_spectrum)(const fxref_config *c, const uint8_t *data,
   FN(ws) w;
   if (FN(ws_init)(&w, c)) { FN(ws_free)(&w); return -1; }
   const size_t L = (size_t)w.d.fftchannels;
-  int ok = FN(station)(&w, c, data, poly, fileoffset, fftindex, w.chan);
+  int ok = FN(station)(&w, c, data, poly, fileoffset, fftindex, w.chan, -1);
   for (int p = 0; p < 2; p++)
     for (int q = 0; q < c->nchan; q++) {
       out[2 * ((size_t)p * c->nchan + q)] = ok ? w.chan[2 * (L * p + q)] : 0;
@@ -371,7 +372,7 @@ static int FN(process_range)(FN(ws) * w, const fxref_config *c,
   for (int i = t0; i < t1; i++) {                      /* :276 */
     for (int j = 0; j < c->nant; j++)                  /* :279 */
       w->valid[j] = FN(station)(w, c, data[j], delays + 3 * (size_t)j,
-                                offsets[j], i, w->chan + 4 * L * (size_t)j);
+                                offsets[j], i, w->chan + 4 * L * (size_t)j, j);
     FN(xmac)(w, c, vis, counts);
   }
   return 0;
