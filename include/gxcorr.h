@@ -138,7 +138,7 @@ gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station);
  * gcorr's ingest (ingest/src/ingest.c:267-355) reads VDIF frames, strips
  * the headers and places each payload by its time stamp:
  *   slot = (seconds - ref_second) * frames_per_second + frame - ref_frame
- * (ingest.c:315).  gxc_set_input_vdif does that on the GPU for one
+ * (ingest.c:313).  gxc_set_input_vdif does that on the GPU for one
  * station of the next subintegration: frames already in device memory,
  * fixed frame size, two channels (the polarisations) per frame, nbit and
  * real/complex as configured.  Header fields follow the VDIF layout that

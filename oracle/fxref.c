@@ -127,7 +127,7 @@ long long fxref_vdif_place(const fxref_config *c, const uint8_t *stream,
   *frame_samples = (int)(payload * 8 / bytes_per_time);
   const long long sb = (long long)d.bytes_per_station;
   const long long nslots = (sb + payload - 1) / payload;
-  const long long nframes = (long long)(nbytes / (size_t)fb); /* whole frames only, ingest.c:303 */
+  const long long nframes = (long long)(nbytes / (size_t)fb); /* whole frames only, ingest.c:304-305 */
   stats->frames = nframes;
   long long *src = (long long *)malloc(sizeof(long long) * (size_t)nslots);
   if (!src) return -1;

@@ -152,7 +152,7 @@ void fxref_xmac_f32(const float *spectra, int nant, int nchan, int nfft,
  * (bits 24-29) frame:24; w2 = log2(channels):5 (bits 24-28) length/8:24;
  * w3 = complex:1 (bit 31) bits-1:5 (26-30) thread:10 (16-25) station:16.
  * Slot of a frame: (seconds - ref_second) * frames_per_second + frame -
- * ref_frame (ingest.c:315).  Frames are read at a fixed stride of
+ * ref_frame (ingest.c:313).  Frames are read at a fixed stride of
  * frame_bytes; checks in order: thread (other_thread), format (malformed:
  * length, legacy bit differing from the first frame, channels != 2, bits,
  * data type, frame >= frames_per_second), invalid bit (invalid), epoch and
