@@ -60,6 +60,14 @@ class VdifStats(ctypes.Structure):
                  "out_of_window", "malformed")]
 
 
+class DifxMeta(ctypes.Structure):
+    _fields_ = [
+        ("mjd", ctypes.c_int), ("seconds", ctypes.c_double), ("config_index", ctypes.c_int),
+        ("source_index", ctypes.c_int), ("freq_index", ctypes.c_int), ("pols", ctypes.c_char * 2),
+        ("uvw", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 class Timings(ctypes.Structure):
     _fields_ = [
         ("h2d_ms", ctypes.c_double), ("fstage_ms", ctypes.c_double),
@@ -141,6 +149,10 @@ def load():
         "gxc_get_row_counts": (ctypes.c_int, [H, ip, ctypes.c_size_t]),
         "gxc_save_visibilities": (ctypes.c_int, [H, ctypes.c_char_p]),
         "gxc_visibilities_device": (vp, [H]),
+        "gxc_save_visibilities_binary": (ctypes.c_int, [H, ctypes.c_char_p]),
+        "gxc_save_difx": (ctypes.c_int, [H, ctypes.c_char_p, P(DifxMeta)]),
+        "gxc_lag_snr": (ctypes.c_int, [H, P(ctypes.c_float), ip, ctypes.c_size_t]),
+        "gxc_lag_snr_series": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_float), ip]),
         "gxc_station_bytes": (ctypes.c_size_t, [P(Config)]),
         "gxc_num_rows": (ctypes.c_int, [P(Config)]),
         "gxc_fstage": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, vp, u8p]),

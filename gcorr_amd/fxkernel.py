@@ -199,6 +199,33 @@ class FxKernel:
     def saveVisibilities(self, path):
         check(self._L.gxc_save_visibilities(self._h, str(path).encode()), self._h)
 
+    def save_binary(self, path):
+        """gxc_save_visibilities_binary (read back with gcorr_amd.visio.read_binary)."""
+        check(self._L.gxc_save_visibilities_binary(self._h, str(path).encode()), self._h)
+
+    def save_difx(self, path, mjd=0, seconds=0.0, pols="XY", uvw=None, config_index=0,
+                  source_index=0, freq_index=0):
+        """gxc_save_difx: DiFX binary visibility records, one per row and product."""
+        m = _lib.DifxMeta()
+        m.mjd, m.seconds = int(mjd), float(seconds)
+        m.config_index, m.source_index, m.freq_index = int(config_index), int(source_index), int(freq_index)
+        m.pols = pols.encode()[:2]
+        keep = None
+        if uvw is not None:
+            keep = np.ascontiguousarray(uvw, np.float64).reshape(self.nrows, 3)
+            m.uvw = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(self._L.gxc_save_difx(self._h, str(path).encode(), ctypes.byref(m)), self._h)
+        del keep
+
+    def lag_snr(self):
+        """gxc_lag_snr: (snr, peak) arrays [rows][4] of the last process()."""
+        n = self.nrows * 4
+        snr = np.empty(n, np.float32)
+        peak = np.empty(n, np.int32)
+        check(self._L.gxc_lag_snr(self._h, snr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  peak.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n), self._h)
+        return snr.reshape(self.nrows, 4), peak.reshape(self.nrows, 4)
+
     # -- extras -----------------------------------------------------------------
     def visibilities(self):
         out = np.empty((self.nrows, 4, self.nchan), np.complex64)
@@ -279,6 +306,21 @@ class FxKernel:
     set_input_vdif = setInputVDIF
     set_delays = setDelays
     save_visibilities = saveVisibilities
+
+
+def lag_snr(vis):
+    """gxc_lag_snr_series on the GPU: vis [..., nchan] complex (nchan a power
+    of two); returns (snr, peak) of shape vis.shape[:-1]."""
+    L = _lib.load()
+    v = np.ascontiguousarray(vis, np.complex64)
+    nchan = v.shape[-1]
+    n = v.size // nchan
+    snr = np.empty(n, np.float32)
+    peak = np.empty(n, np.int32)
+    check(L.gxc_lag_snr_series(v.ctypes.data_as(ctypes.c_void_p), n, nchan,
+                               snr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               peak.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return snr.reshape(v.shape[:-1]), peak.reshape(v.shape[:-1])
 
 
 def xengine(spectra, nant, autos=False, precision="fp32"):

@@ -228,6 +228,43 @@ gxc_status gxc_get_row_counts(gxc_handle *h, int *counts, size_t n);
  * format, one line per channel. */
 gxc_status gxc_save_visibilities(gxc_handle *h, const char *path);
 
+/* ---- Visibility output and the fringe check ------------------------------
+ * Binary visibilities of the last gxc_process, little-endian:
+ *   char  magic[8] = "GXCVIS01"
+ *   int32 version (1), nant, nchan, nrows, nprod (4), autos, numffts, precision
+ *   f64   bandwidth_hz, lo_hz
+ *   int32 stations[nrows][2]   (j, k) of each row; autos have j == k
+ *   int32 counts[nrows]        FFTs summed per row (fxkernel's baselineCount)
+ *   cf32  vis[nrows][4][nchan] the [baseline][product][channel] layout
+ * The text of gxc_save_visibilities (fxkernel.cpp:370-412) in binary, with
+ * the metadata needed to read it back. */
+gxc_status gxc_save_visibilities_binary(gxc_handle *h, const char *path);
+
+/* DiFX-style output: one record per (row, product) in DiFX's binary
+ * visibility format: a 74-byte header (sync word 0xFF00FF00, version 1,
+ * baseline 256*(j+1)+(k+1), MJD, seconds, config, source and frequency
+ * indices, the two polarisation names, pulsar bin 0, data weight =
+ * count/numffts, u, v, w in metres) followed by nchan cf32 values. */
+typedef struct {
+  int mjd;              /* day of the subintegration */
+  double seconds;       /* seconds into that day */
+  int config_index, source_index, freq_index;
+  char pols[2];         /* names of polarisations 0 and 1, e.g. 'X','Y' or 'R','L' */
+  const double *uvw;    /* [rows][3] in metres, or NULL for zeros */
+} gxc_difx_meta;
+gxc_status gxc_save_difx(gxc_handle *h, const char *path, const gxc_difx_meta *meta);
+
+/* Lag S/N of the last gxc_process, the fringe check of util/lagsnr.py:33-36:
+ * for series i = row*4 + product, snr[i] = max|ifft(V)| / std|ifft(V)|
+ * (population standard deviation over the nchan lags) and peak[i] = the lag
+ * index of the maximum (the first one on ties).  n >= rows*4.  Runs on the
+ * GPU (radix-2 Stockham inverse FFT of every series, then one reduction per
+ * series). */
+gxc_status gxc_lag_snr(gxc_handle *h, float *snr, int *peak, size_t n);
+/* The same for nseries host series of nchan (a power of two) visibilities,
+ * on the current device. */
+gxc_status gxc_lag_snr_series(const gxc_cf32 *vis, int nseries, int nchan, float *snr, int *peak);
+
 /* Device pointer of the visibilities [rows][4][nchan] cf32 (stream-ordered). */
 const void *gxc_visibilities_device(const gxc_handle *h);
 
