@@ -135,6 +135,9 @@ def load():
         "gxc_set_input": (ctypes.c_int, [H, P(u8p)]),
         "gxc_set_input_device": (ctypes.c_int, [H, P(vp)]),
         "gxc_set_delays": (ctypes.c_int, [H, P(dp), dp]),
+        "gxc_set_pol_conversion": (ctypes.c_int, [H, vp]),
+        "gxc_requantise_2bit": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.c_float, vp, vp]),
+        "gxc_requantise_8bit": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.c_float, vp, vp]),
         "gxc_vdif_spec_init": (None, [P(VdifSpec)]),
         "gxc_vdif_slots": (ctypes.c_longlong, [P(Config), ctypes.c_int, ctypes.c_int]),
         "gxc_set_input_vdif": (ctypes.c_int, [H, ctypes.c_int, vp, ctypes.c_size_t, P(VdifSpec),

@@ -27,6 +27,7 @@
 #include "gxc_host.hpp"
 #include "gxc_internal.hpp"
 #include "gxc_lags.hpp"
+#include "gxc_requant.hpp"
 #include "gxc_vdif.hpp"
 #include "gxc_xengine.hpp"
 
@@ -69,6 +70,10 @@ struct gxc_handle {
   unsigned *d_slot_src = nullptr;         // [slot_cap] ingest scratch
   long long slot_cap = 0;
   unsigned long long *d_vstats = nullptr;  // [kVdifCounters]
+
+  // polarisation conversion of the visibilities (gxc_set_pol_conversion)
+  bool polconv = false;
+  float2 *d_polgains = nullptr;  // [nant][2][nchan]
 
   // delays
   bool delays_set = false;
@@ -286,6 +291,8 @@ gxc_status enqueue_device_work(gxc_handle *h) {
   }
   mark(h, 4, xs);
   GXC_TRY(run_finalise(h, xs));
+  if (h->polconv)
+    GXC_HIP(h, gxc::launch_polconv(h->d_vis, h->d_polgains, h->d_rowmap, h->d.nrows, c.nchan, xs));
   mark(h, 5, xs);
   if (overlap) {
     GXC_HIP(h, hipEventRecord(h->ev_join, xs));
@@ -536,6 +543,7 @@ void gxc_destroy(gxc_handle *h) {
     if (e) hipEventDestroy(e);
   for (uint8_t *f : h->d_flags) hipFree(f);
   hipFree(h->d_flag_ptrs);
+  hipFree(h->d_polgains);
   hipFree(h->d_spf);
   hipFree(h->d_nslots);
   hipFree(h->d_slot_src);
@@ -746,6 +754,48 @@ gxc_status gxc_get_frame_flags(gxc_handle *h, int station, uint8_t *out, size_t 
   if (!out) return GXC_OK;
   if (n < (size_t)ns) return set_err(h, GXC_ERR_INVALID_ARG, "flag buffer too small");
   GXC_HIP(h, hipMemcpy(out, h->d_flags[station], (size_t)ns, hipMemcpyDeviceToHost));
+  return GXC_OK;
+}
+
+namespace {
+gxc_status requant_args(const float *a, const float *b, const void *out, size_t outalign) {
+  if (!a || !b || !out) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null buffer");
+  if (reinterpret_cast<uintptr_t>(a) % 16 || reinterpret_cast<uintptr_t>(b) % 16 ||
+      reinterpret_cast<uintptr_t>(out) % outalign)
+    return set_err(nullptr, GXC_ERR_INVALID_ARG, "misaligned requantiser buffer");
+  return GXC_OK;
+}
+}  // namespace
+
+gxc_status gxc_requantise_2bit(const float *d_pol0, const float *d_pol1, size_t nsamples, float threshold,
+                               uint8_t *d_out, void *stream) {
+  GXC_TRY(requant_args(d_pol0, d_pol1, d_out, 8));
+  if (nsamples % 2) return set_err(nullptr, GXC_ERR_INVALID_ARG, "2-bit requantiser needs an even sample count");
+  if (!(threshold > 0.f)) return set_err(nullptr, GXC_ERR_INVALID_ARG, "threshold must be positive");
+  GXC_HIP(nullptr, gxc::launch_requant2(d_pol0, d_pol1, (long long)nsamples, threshold, d_out,
+                                        static_cast<hipStream_t>(stream)));
+  return GXC_OK;
+}
+
+gxc_status gxc_requantise_8bit(const float *d_pol0, const float *d_pol1, size_t nsamples, float scale,
+                               uint8_t *d_out, void *stream) {
+  GXC_TRY(requant_args(d_pol0, d_pol1, d_out, 16));
+  GXC_HIP(nullptr, gxc::launch_requant8(d_pol0, d_pol1, (long long)nsamples, scale, d_out,
+                                        static_cast<hipStream_t>(stream)));
+  return GXC_OK;
+}
+
+gxc_status gxc_set_pol_conversion(gxc_handle *h, const gxc_cf32 *gains) {
+  if (!h) return set_err(nullptr, GXC_ERR_INVALID_ARG, "null handle");
+  GXC_HIP(h, hipStreamSynchronize(h->stream));
+  const bool on = gains != nullptr;
+  if (on) {
+    const size_t n = (size_t)h->cfg.nant * 2 * h->cfg.nchan;
+    if (!h->d_polgains) GXC_TRY(dmalloc(h, &h->d_polgains, n));
+    GXC_HIP(h, hipMemcpy(h->d_polgains, gains, n * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  if (on != h->polconv) destroy_graph(h);  // the captured work changes
+  h->polconv = on;
   return GXC_OK;
 }
 

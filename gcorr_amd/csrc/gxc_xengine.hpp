@@ -60,6 +60,10 @@ int xm_tiles(int M, int out_mode, int *out);
 hipError_t launch_scales(float *scales, const float *power, int n, int nrows, float target, hipStream_t s);
 constexpr int kPowerRows = 64;  // FFTs sampled for the scale estimate
 hipError_t launch_finalise_mixed(const FinMParams &p, hipStream_t s);
+// Polarisation conversion of finished visibilities [nrows][4][nchan]:
+// V' = J_j V J_k^H with J = [[g0, -g1], [g0, g1]] from gains [nant][2][nchan].
+hipError_t launch_polconv(float2 *vis, const float2 *gains, const int2 *rowmap, int nrows, int nchan,
+                          hipStream_t s);
 // acc[i] += v[i] for n floats.
 hipError_t launch_add(float *acc, const float *v, size_t n, hipStream_t s);
 // Fill n floats with v (a kernel, so it may sit inside a captured graph).

@@ -161,6 +161,15 @@ class FxKernel:
                                           None), self._h)
         return out
 
+    def set_pol_conversion(self, gains):
+        """gains: complex [nant][2][nchan] (g0, g1 per station and channel) or
+        None to turn the conversion off (gxc_set_pol_conversion)."""
+        if gains is None:
+            check(self._L.gxc_set_pol_conversion(self._h, None), self._h)
+            return
+        g = np.ascontiguousarray(gains, np.complex64).reshape(self.nant, 2, self.nchan)
+        check(self._L.gxc_set_pol_conversion(self._h, g.ctypes.data_as(ctypes.c_void_p)), self._h)
+
     def setDelays(self, d, f):
         """d: [nant][3] polynomial (s, t in FFTs), f: [nant] file offsets (s)."""
         d = np.ascontiguousarray(d, dtype=np.float64).reshape(self.nant, 3)
@@ -306,6 +315,21 @@ class FxKernel:
     set_input_vdif = setInputVDIF
     set_delays = setDelays
     save_visibilities = saveVisibilities
+
+
+def requantise_2bit(pol0_ptr, pol1_ptr, nsamples, threshold, out_ptr, stream=0):
+    """gxc_requantise_2bit on device pointers (integers, e.g. tensor.data_ptr());
+    stream: a hipStream_t as an integer (0: the default stream)."""
+    check(_lib.load().gxc_requantise_2bit(ctypes.c_void_p(int(pol0_ptr)), ctypes.c_void_p(int(pol1_ptr)),
+                                          int(nsamples), float(threshold), ctypes.c_void_p(int(out_ptr)),
+                                          ctypes.c_void_p(int(stream)) if stream else None))
+
+
+def requantise_8bit(pol0_ptr, pol1_ptr, nsamples, scale, out_ptr, stream=0):
+    """gxc_requantise_8bit on device pointers (complex samples, re/im interleaved)."""
+    check(_lib.load().gxc_requantise_8bit(ctypes.c_void_p(int(pol0_ptr)), ctypes.c_void_p(int(pol1_ptr)),
+                                          int(nsamples), float(scale), ctypes.c_void_p(int(out_ptr)),
+                                          ctypes.c_void_p(int(stream)) if stream else None))
 
 
 def lag_snr(vis):

@@ -134,6 +134,34 @@ gxc_status gxc_set_input(gxc_handle *h, const uint8_t *const *per_station);
  * gcorr/src/testgpukernel.cu:468-470). */
 gxc_status gxc_set_input_device(gxc_handle *h, const void *const *per_station);
 
+/* Polarisation conversion (e.g. linear to circular), as gpubench's
+ * linear2circular_kernel (examples/chris/bench/gpubench.cu:141-166) applies it
+ * to each station's channelised spectra: y0 = g0 x0 - g1 x1,
+ * y1 = g0 x0 + g1 x1 with complex gains g0, g1 per station and channel.
+ * gains is [nant][2][nchan] (g0 then g1 of each station), copied at this
+ * call; NULL turns the conversion off.  The result equals converting every
+ * spectrum; it is computed on the visibilities as V' = J_j V J_k^H with
+ * J = [[g0, -g1], [g0, g1]], which is the same sum.  The row counts are
+ * unchanged. */
+gxc_status gxc_set_pol_conversion(gxc_handle *h, const gxc_cf32 *gains);
+
+/* Requantisers: float voltages in device memory -> the packed station
+ * layout (gpubench packs filterbank output to 2 bits on the GPU,
+ * examples/chris/bench/gpubench.cu:64-130).  The 2-bit codes are the ones
+ * fxkernel's LUT decodes (fxkernel.cpp:15-28, 0..3 -> -HI, -1, +1, +HI):
+ *   x < -t -> 0, -t <= x < 0 -> 1, 0 <= x < t -> 2, x >= t -> 3 (NaN -> 1),
+ * packed LSB first [A_t, B_t, A_t+1, B_t+1] (gpubench's own pack2bit uses
+ * the reverse code order); nsamples even, output nsamples/2 bytes.
+ * 8-bit: complex samples (re, im interleaved floats) to
+ * q = clamp(rint(x*scale), -127, 127) (NaN -> 0), bytes [Are, Aim, Bre, Bim],
+ * output 4*nsamples bytes.  pol0/pol1 16-byte aligned; out 8- (2-bit) or
+ * 16-byte (8-bit) aligned.  Queued on `stream` (a hipStream_t, NULL for the
+ * default stream); the call does not wait. */
+gxc_status gxc_requantise_2bit(const float *d_pol0, const float *d_pol1, size_t nsamples, float threshold,
+                               uint8_t *d_out, void *stream);
+gxc_status gxc_requantise_8bit(const float *d_pol0, const float *d_pol1, size_t nsamples, float scale,
+                               uint8_t *d_out, void *stream);
+
 /* ---- VDIF ingest --------------------------------------------------------
  * gcorr's ingest (ingest/src/ingest.c:267-355) reads VDIF frames, strips
  * the headers and places each payload by its time stamp:

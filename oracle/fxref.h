@@ -26,6 +26,7 @@
  *                stride law (fxkernel.cpp:105-114,553-558).
  *   nchan that is not a perfect square: stride s = 2^floor(log2(nchan)/2),
  *                nchan/s steps (fxkernel.cpp:58-63 refuses such nchan).
+ *   polgains     per-spectrum polarisation conversion (gpubench.cu:141-166).
  *   frame_ok     missing-frame flags from a VDIF ingest: an FFT whose input
  *                samples touch a missing frame is invalid for its station
  *                (fxkernel has no missing data; gcorr's ingest,
@@ -57,6 +58,11 @@ typedef struct {
    * is present. */
   const uint8_t *const *frame_ok;
   const int *frame_samples;
+  /* Polarisation conversion of every spectrum (NULL: none), as
+   * examples/chris/bench/gpubench.cu:141-166 applies it after the
+   * filterbank: y0 = g0 x0 - g1 x1, y1 = g0 x0 + g1 x1 per channel, with
+   * polgains [nant][2][nchan] complex (interleaved re, im). */
+  const float *polgains;
 } fxref_config;
 
 typedef struct {

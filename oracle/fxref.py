@@ -29,6 +29,7 @@ class Config(ctypes.Structure):
         ("quirk", ctypes.c_int),
         ("frame_ok", ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))),
         ("frame_samples", ctypes.POINTER(ctypes.c_int)),
+        ("polgains", ctypes.POINTER(ctypes.c_float)),
     ]
 
 
@@ -114,18 +115,26 @@ def _ptr(a, ct):
 
 
 def make_config(nant, nchan, numffts, lo, bw, nbit=2, is_complex=False, autos=False,
-                quirk=True, frame_ok=None, frame_samples=None):
+                quirk=True, frame_ok=None, frame_samples=None, polgains=None):
     """frame_ok: per-station uint8 flags of a VDIF ingest, with
-    frame_samples[station] time samples per frame (None: no missing data)."""
+    frame_samples[station] time samples per frame (None: no missing data).
+    polgains: complex [nant][2][nchan] polarisation-conversion gains (None: none)."""
     c = Config(int(nant), int(nchan), int(numffts), int(nbit), int(bool(is_complex)),
                float(lo), float(bw), int(bool(autos)), int(bool(quirk)))
+    keep = []
+    if polgains is not None:
+        g = np.ascontiguousarray(np.asarray(polgains, np.complex64).reshape(int(nant), 2, int(nchan)))
+        gf = g.view(np.float32)
+        c.polgains = _ptr(gf, ctypes.c_float)
+        keep.append(gf)
+        c._keep = (keep,)
     if frame_ok is not None:
         flags = [np.ascontiguousarray(f, np.uint8) for f in frame_ok]
         spf = np.ascontiguousarray(frame_samples, np.int32).reshape(int(nant))
         ptrs = (ctypes.POINTER(ctypes.c_uint8) * int(nant))(*[_ptr(f, ctypes.c_uint8) for f in flags])
         c.frame_ok = ctypes.cast(ptrs, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)))
         c.frame_samples = _ptr(spf, ctypes.c_int)
-        c._keep = (flags, spf, ptrs)
+        c._keep = (flags, spf, ptrs, keep)
     return c
 
 

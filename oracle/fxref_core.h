@@ -297,6 +297,21 @@ static int FN(station)(FN(ws) * w, const fxref_config *c, const uint8_t *data,
       x[2 * q + 1] = fr * xi + fi * xr;
     }
   }
+  if (c->polgains && st >= 0) { /* gpubench.cu:156-166 */
+    const float *g0 = c->polgains + 2 * (size_t)c->nchan * (2 * (size_t)st);
+    const float *g1 = g0 + 2 * (size_t)c->nchan;
+    REAL *x0 = chan_j, *x1 = chan_j + 2 * L;
+    for (int q = 0; q < c->nchan; q++) {
+      const REAL ar = (REAL)g0[2 * q], ai = (REAL)g0[2 * q + 1];
+      const REAL br = (REAL)g1[2 * q], bi = (REAL)g1[2 * q + 1];
+      const REAL ur = ar * x0[2 * q] - ai * x0[2 * q + 1], ui = ar * x0[2 * q + 1] + ai * x0[2 * q];
+      const REAL vr = br * x1[2 * q] - bi * x1[2 * q + 1], vi = br * x1[2 * q + 1] + bi * x1[2 * q];
+      x0[2 * q] = ur - vr;
+      x0[2 * q + 1] = ui - vi;
+      x1[2 * q] = ur + vr;
+      x1[2 * q + 1] = ui + vi;
+    }
+  }
   return 1;
 }
 
