@@ -306,6 +306,32 @@ def main():
         kv.close()
         del frames
 
+    # Requantiser (gxc_requantise_2bit): one station's float voltages of this
+    # workload (both pols) -> its packed 2-bit buffer, HIP events on the
+    # stream it runs on (torch's current stream).
+    rq = None
+    if not args.no_vdif and not w.is_complex and args.workload != "c4":
+        n = 2 * w.station_bytes  # time samples per pol
+        x = torch.randn(2, n, device=dev)
+        out = torch.empty(w.station_bytes, dtype=torch.uint8, device=dev)
+        stream = torch.cuda.current_stream()
+        for _ in range(2):
+            gcorr_amd.requantise_2bit(x[0].data_ptr(), x[1].data_ptr(), n, 0.98, out.data_ptr(),
+                                      stream.cuda_stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record(stream)
+        for _ in range(reps):
+            gcorr_amd.requantise_2bit(x[0].data_ptr(), x[1].data_ptr(), n, 0.98, out.data_ptr(),
+                                      stream.cuda_stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 2 * n * 4 + w.station_bytes
+        rq = {"samples_per_pol": n, "ms": ms, "GBps": nbytes / (ms * 1e-3) / 1e9,
+              "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del x, out
+
     cpu = None
     if not args.no_cpu_baseline and args.workload != "c4":
         cpu = cpu_baseline(w, poly, offs, args.cpu_seconds)
@@ -341,6 +367,7 @@ def main():
         "gbps_input": value * (16 if w.is_complex else 2),
         "pcie_inclusive": pcie,
         "vdif": vd,
+        "requantise_2bit": rq,
         "cpu_baseline": cpu,
     }
     k.close()
