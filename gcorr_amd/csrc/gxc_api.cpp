@@ -508,7 +508,9 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   // launch shapes: given, or defaults
   {
     int fb[4], nfb = gxc::fstage_blocks(d.L, cfg->is_complex != 0, fb);
-    bool ok = cfg->fblock == 0;
+    bool ok = cfg->fblock == 0 ||
+              (cfg->fblock == gxc::kFBlockMfma && out_mode != gxc::OUT_F32 &&
+               gxc::fstage_mfma_supported(d.L, cfg->is_complex != 0));
     for (int i = 0; i < nfb; i++) ok = ok || cfg->fblock == fb[i];
     if (!ok) return fail(set_err(h, GXC_ERR_INVALID_ARG, "fblock is not an F-stage workgroup size for this FFT length"));
     int xt[3], nxt = gxc::xm_tiles(d.M, out_mode, xt);
@@ -1160,7 +1162,9 @@ gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out) {
   // F-stage: FFT blocks per workgroup
   double best_f = 1e30;
   int best_fb = h->fblock;
-  int fb[4], nfb = gxc::fstage_blocks(h->d.L, h->cfg.is_complex != 0, fb);
+  int fb[5], nfb = gxc::fstage_blocks(h->d.L, h->cfg.is_complex != 0, fb);
+  if (h->out_mode != gxc::OUT_F32 && gxc::fstage_mfma_supported(h->d.L, h->cfg.is_complex != 0))
+    fb[nfb++] = gxc::kFBlockMfma;
   for (int i = 0; i < nfb && st == GXC_OK; i++) {
     h->fblock = fb[i];
     double ms = 0;

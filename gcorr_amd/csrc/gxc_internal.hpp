@@ -93,6 +93,11 @@ int fstage_blocks(int L, bool cplx, int *out /* [4] */);
 // block: one of fstage_blocks (0 = default); ignored for L > 8192
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, int block, hipStream_t s);
 hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s);  // L = 16384, 32768
+// F-stage with the FFT passes on the matrix cores (fstage_mfma.hip): real
+// input, L = 2048..8192, reduced-precision outputs (and the power pass).
+constexpr int kFBlockMfma = -1;
+bool fstage_mfma_supported(int L, bool cplx);
+hipError_t launch_fstage_mfma(const FParams &p, int L, hipStream_t s);
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
 // Multi-subband F-stage: real 2-bit input, FFT lengths 256..8192, workgroups
 // of `block` threads (256, the default, or 512); units of a subband =

@@ -87,11 +87,16 @@ typedef struct {
                       hipGraph on first use and replay it afterwards */
   /* Launch shapes (0 = default; gxc_autotune measures and sets them): */
   int fblock;      /* F-stage workgroup size, i.e. FFT blocks per workgroup
-                      = fblock / (fftchannels/16) */
+                      = fblock / (fftchannels/16); or GXC_FBLOCK_MFMA */
   int xtile;       /* X-stage channel tile (channels per workgroup) */
   int xsplits;     /* X-stage FFT splits (workgroups per channel tile) */
   int reserved[1];
 } gxc_config;
+
+/* gxc_config.fblock value selecting the F-stage whose FFT passes run on the
+ * matrix cores (f16 operands, FP32 accumulation): reduced precision only,
+ * real input, fftchannels 2048..8192. */
+#define GXC_FBLOCK_MFMA (-1)
 
 typedef struct gxc_handle gxc_handle;
 
