@@ -49,6 +49,26 @@ class FxKernel {
   void accumulate(cf32 ***odata) { check(gxc_accumulate(h_, odata), h_); }
   void saveVisibilities(const char *outfile) { check(gxc_save_visibilities(h_, outfile), h_); }
 
+  // Extensions beyond fxkernel.h, thin wrappers of the C-ABI calls of the
+  // same names (see gxcorr.h for their contracts).
+  void setInputDevice(const void *const *dev) { check(gxc_set_input_device(h_, dev), h_); }
+  gxc_vdif_stats setInputVDIF(int station, const void *d_frames, size_t nbytes, const gxc_vdif_spec &spec) {
+    gxc_vdif_stats st;
+    check(gxc_set_input_vdif(h_, station, d_frames, nbytes, &spec, &st), h_);
+    return st;
+  }
+  void setPolConversion(const cf32 *gains) { check(gxc_set_pol_conversion(h_, gains), h_); }
+  void integrateBegin() { check(gxc_integrate_begin(h_), h_); }
+  void submit(u8 **idata, double **d, double *f) { check(gxc_submit(h_, idata, d, f), h_); }
+  int integrateEnd(cf32 *out, size_t nelem) {
+    int n = 0;
+    check(gxc_integrate_end(h_, out, nelem, &n), h_);
+    return n;
+  }
+  void saveVisibilitiesBinary(const char *outfile) { check(gxc_save_visibilities_binary(h_, outfile), h_); }
+  void saveDiFX(const char *outfile, const gxc_difx_meta &meta) { check(gxc_save_difx(h_, outfile, &meta), h_); }
+  void lagSNR(float *snr, int *peak, size_t n) { check(gxc_lag_snr(h_, snr, peak, n), h_); }
+
   gxc_handle *handle() { return h_; }
 
  private:
