@@ -143,3 +143,50 @@ def test_stream_state_errors(gpu):
     got, n = k.integrate_end()  # nothing submitted: zero sum
     assert n == 0 and not np.any(got)
     k.close()
+
+
+def test_stream_with_pol_conversion(gpu):
+    w = synth.Workload("s", nant=3, nchan=256, numffts=64, bw=32e6, lo=1.65e9, seed=12)
+    subs = subints(w, 3, seed=13)
+    rng = np.random.default_rng(14)
+    g = ((rng.standard_normal((3, 2, 256)) + 1j * rng.standard_normal((3, 2, 256))) / 2).astype(np.complex64)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True)
+    k.set_pol_conversion(g)
+    expect = ordered_sum(individual(k, subs))
+    k.integrate_begin()
+    for data, poly, offs in subs:
+        k.submit(data, poly, offs)
+    got, n = k.integrate_end()
+    k.close()
+    assert n == 3
+    np.testing.assert_array_equal(got, expect)
+
+
+def test_stream_after_vdif_input(gpu):
+    """A VDIF ingest's frame flags do not leak into submitted subintegrations,
+    and the handle needs new input afterwards."""
+    import torch
+    from gcorr_amd import vdif
+    w = synth.Workload("s", nant=3, nchan=128, numffts=64, bw=16e6, lo=1.65e9, seed=15)
+    subs = subints(w, 2, seed=16)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw)
+    expect = ordered_sum(individual(k, subs))
+    data0, poly0, offs0 = subs[0]
+    devs = []
+    for a in range(w.nant):
+        fr = vdif.frames_from_station(data0[a], 256, frames_per_second=10).reshape(-1)
+        fr = fr[: fr.size - fr.size // 2]  # drop the second half of the frames
+        dev = torch.from_numpy(np.ascontiguousarray(fr)).to("cuda")
+        devs.append(dev)
+        k.setInputVDIF(a, dev.data_ptr(), fr.size, frames_per_second=10, keepalive=dev)
+    k.setDelays(poly0, offs0)
+    k.process()
+    assert k.row_counts().max() < w.numffts  # the missing frames took FFTs out
+    k.integrate_begin()
+    for data, poly, offs in subs:
+        k.submit(data, poly, offs)
+    got, n = k.integrate_end()
+    np.testing.assert_array_equal(got, expect)
+    with pytest.raises(gcorr_amd.GxcError):  # the VDIF input was released
+        k.process()
+    k.close()
