@@ -266,26 +266,40 @@ __device__ __forceinline__ uint16_t pack_fp8(float re, float im) {
   return (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false) & 0xffff);
 }
 
-// Store bin q of output row `row` (= 2*station + pol within FFT row block)
-// in the spectra format OUT.  `srow` is the input index 2*station+pol.
+// Scale of bin q of input srow (= 2*station+pol) for the reduced-precision
+// forms; 1 for the others.
 template <int OUT, int NCHAN>
-__device__ __forceinline__ void store_bin(const FParams &p, size_t row, int srow, int q, float2 y) {
+__device__ __forceinline__ float bin_scale(const FParams &p, int srow, int q) {
+  if constexpr (OUT == OUT_F32 || OUT == OUT_POWER) {
+    return 1.f;
+  } else {
+    return p.scales[(size_t)srow * NCHAN + q];
+  }
+}
+
+// Store bin q of output row `row` (= 2*station + pol within FFT row block)
+// in the spectra format OUT, with its scale sc = bin_scale(...) already read.
+template <int OUT, int NCHAN>
+__device__ __forceinline__ void store_bin_sc(const FParams &p, size_t row, int q, float2 y, float sc) {
   if constexpr (OUT == OUT_F32) {
     p.spectra[row * NCHAN + q] = y;
   } else if constexpr (OUT == OUT_POWER) {
     // one row per sampled FFT; the scale kernel sums rows in a fixed order,
     // so the scales do not depend on scheduling
     p.power[row * NCHAN + q] = y.x * y.x + y.y * y.y;
+  } else if constexpr (OUT == OUT_FP8) {
+    static_cast<uint16_t *>(p.out)[row * NCHAN + q] = pack_fp8(y.x * sc, y.y * sc);
+  } else if constexpr (OUT == OUT_F16) {
+    static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_f16(y.x * sc, y.y * sc);
   } else {
-    const float sc = p.scales[(size_t)srow * NCHAN + q];
-    if constexpr (OUT == OUT_FP8) {
-      static_cast<uint16_t *>(p.out)[row * NCHAN + q] = pack_fp8(y.x * sc, y.y * sc);
-    } else if constexpr (OUT == OUT_F16) {
-      static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_f16(y.x * sc, y.y * sc);
-    } else {
-      static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_bf16(y.x * sc, y.y * sc);
-    }
+    static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_bf16(y.x * sc, y.y * sc);
   }
+}
+
+// Store bin q of output row `row`; `srow` is the input index 2*station+pol.
+template <int OUT, int NCHAN>
+__device__ __forceinline__ void store_bin(const FParams &p, size_t row, int srow, int q, float2 y) {
+  store_bin_sc<OUT, NCHAN>(p, row, q, y, bin_scale<OUT, NCHAN>(p, srow, q));
 }
 
 // Workgroups are dealt to the 8 XCDs round robin in dispatch order, each XCD

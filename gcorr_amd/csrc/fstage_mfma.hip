@@ -68,7 +68,7 @@ struct MG {
   static constexpr int NW = NTHR / 64;              // waves per workgroup
   static constexpr int K01 = L / 256 / NW;          // 16-column tiles per wave, passes 0 and 1
   static constexpr int K2 = O / NW;                 // 16-column tiles per wave, last pass
-  static constexpr int PS = L + L / 8;              // padded words per LDS buffer
+  static constexpr int PS = L + L / 8;              // padded words of the LDS buffer
   static constexpr int PPU = L >= 8192 ? 1 : 2;     // polarisations per unit
   using TG = TabGeom<L, false>;
   static_assert(L >= 2048 && L <= 8192, "MFMA F-stage covers L = 2048..8192");
@@ -125,44 +125,55 @@ __device__ __forceinline__ void dft_tile(const uint4 &rh, const uint4 &rl, const
 }
 
 template <int L, int OUT>
-__global__ void __launch_bounds__(MG<L>::NTHR) fstage_mfma_kernel(FParams p, long long nunits) {
+__global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, long long nunits) {
   using G = MG<L>;
   using TG = typename G::TG;
   constexpr int NW = G::NW, K01 = G::K01, K2 = G::K2, RL = G::RL, O = G::O, S = G::S, LQ = G::LQ;
-  constexpr int NCHAN = TG::NCHAN, S2 = TG::S2, NROT = TG::S2 + TG::NSTEPS;
-  __shared__ __align__(16) uint32_t sd[2 * G::PS];  // buffers A and B: complex f16
-  __shared__ float2 stab[TG::NTAB];                 // rotator tables (fill_tables)
-  __shared__ float slev[4];                         // 2-bit levels (fxkernel.cpp:15-28)
+  constexpr int NCHAN = TG::NCHAN, S2 = TG::S2, NROT = TG::S2 + TG::NSTEPS, TS = TG::S;
+  __shared__ __align__(16) uint32_t sd[G::PS];  // the transform: complex f16, padded
+  __shared__ __align__(16) uint32_t st0[256];   // pass-0 twiddles, [s][r]
+  __shared__ __align__(16) uint32_t st1[L];     // pass-1 twiddles, [t][c][g][i]
+  __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
+  __shared__ float slev[4];                     // 2-bit levels (fxkernel.cpp:15-28)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c16 = lane & 15, g = lane >> 4;
 
-  // Per-launch constants of this lane: DFT matrices and twiddles.
+  // Per-launch constants of this lane: the DFT matrices.
   uint4 d_rh, d_rl, d_ih, d_il;  // DFT-16 of passes 0 and 1
   dft_operand<16, 16>(c16, g, 0, d_rh, d_rl, d_ih, d_il);
-  uint4 l_rh[2], l_rl[2], l_ih[2], l_il[2];  // pruned DFT-RL of the last pass, K-steps 0 and 1
-#pragma unroll
-  for (int ks = 0; ks < 2; ks++) dft_operand<RL, O>(c16, g, ks, l_rh[ks], l_rl[ks], l_ih[ks], l_il[ks]);
-  uint32_t tw0[K01][4], tw1[K01][4];
-#pragma unroll
-  for (int k = 0; k < K01; k++) {
-    const int bc = 16 * (wave + NW * k) + c16;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      // pass-0 output q0 = 16 b + r' feeds pass 1 as element q0 / LQ of
-      // butterfly q0 % LQ: twiddle W_L^((q0 % 16) (L/256) (q0 / LQ))
-      const int q0 = 16 * bc + 4 * g + i;
-      tw0[k][i] = twh<L>(((q0 & 15) * (L / 256)) * (q0 / LQ));
-      // pass-1 output q1 feeds the last pass as element q1 / 256 of
-      // butterfly q1 % 256: twiddle W_L^((q1 % 256) (q1 / 256))
-      const int q1 = (bc >> 4) * 256 + (bc & 15) + 16 * (4 * g + i);
-      tw1[k][i] = twh<L>((q1 & 255) * (q1 >> 8));
-    }
-  }
+  // Pruned DFT-RL of the last pass.  Its two K-steps need only one operand
+  // per lane: for RL <= 16 a row's butterfly lies in K-step row / 8 and its
+  // rows are zero in the other, so the lane keeps the K-step of its own row
+  // (rows >= 8 take the product with the second K-step's inputs); for
+  // RL = 32 the second K-step's matrix is the first with row r scaled by
+  // W_32^(16 r) = (-1)^r.
+  uint4 l_rh, l_rl, l_ih, l_il;
+  dft_operand<RL, O>(c16, g, RL == 32 ? 0 : c16 >> 3, l_rh, l_rl, l_ih, l_il);
+
+  // Twiddle tables, the same for every unit (f16 pairs, widened exactly in
+  // twmul).  Pass-0 output row r of the column tile holding column b takes
+  // W_L^(r RL (b / RL)) = W_256^(r s), s = b / RL < 16: st0[16 s + r].
+  // Pass-1 output q1 = 256 t + c + 16 r (tile t, column c, row r = 4 g + i)
+  // takes W_L^((c + 16 r) t): st1[256 t + 16 c + r].
+  for (int e = tid; e < 256; e += G::NTHR) st0[e] = twh<256>((e & 15) * (e >> 4));
+  for (int e = tid; e < L; e += G::NTHR) st1[e] = twh<L>((((e >> 4) & 15) + 16 * (e & 15)) * (e >> 8));
   if (tid < 4) slev[tid] = tid == 0 ? -kHiMag : tid == 1 ? -1.0f : tid == 2 ? 1.0f : kHiMag;
 
+  // LDS words of this lane's first tile.  pad(x + 32 m) = pad(x) + 36 m, so
+  // the other tiles, rows and K-steps sit at constant offsets from these.
+  const int p0w = pad(256 * wave + 16 * c16 + 4 * g);  // pass-0 store
+  const int p1r = pad(16 * wave + c16 + LQ * 4 * g);   // pass-1 loads
+  const int p1w = pad(256 * wave + c16 + 64 * g);      // pass-1 stores
+  // last-pass loads: input kc = 4 g + j (+ 16 ks) of butterfly b of column C
+  // is point S C + kc / RL + 256 (kc % RL)
+  const int p2r = pad(S * (16 * wave + c16) + (4 * g) / RL + 256 * ((4 * g) % RL));
+  // last-pass bins q = qb + 256 i (+ 16 S NW per tile): row 4 g + i of
+  // column C is output r' = (4 g + i) % O of butterfly (4 g + i) / O
+  const int qb0 = S * (16 * wave + c16) + (4 * g) / O + 256 * ((4 * g) % O);
+
   const float2 *fsub = stab + NROT;
-  const float2 *fstep = fsub + TG::S;
+  const float2 *fstep = fsub + TS;
 
   for (long long u = blockIdx.x; u < nunits; u += gridDim.x) {
     const int item = (int)(G::PPU == 2 ? u : u >> 1);
@@ -175,7 +186,20 @@ __global__ void __launch_bounds__(MG<L>::NTHR) fstage_mfma_kernel(FParams p, lon
     // the same for every thread: the branches below are uniform
     const bool valid = split_offset<L>(p, st, t, offset, frac, a, b);
     if (tid == 0 && pol0 == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
-    __syncthreads();  // the previous unit is done with LDS
+    // This lane's input bytes (both polarisations) for every pass-0 tile:
+    // window samples n = b + LQ r, r = 4 g + j, two samples to a byte.  All
+    // are requested here so that their latency overlaps the table build.
+    uint32_t raw[K01][4];
+    if (valid) {
+      gbyte *src = (gbyte *)p.station[st];
+#pragma unroll
+      for (int k = 0; k < K01; k++) {
+        gbyte *bp = src + ((offset + 16 * (wave + NW * k) + c16) >> 1) + (LQ / 2) * 4 * g;
+#pragma unroll
+        for (int j = 0; j < 4; j++) raw[k][j] = bp[(LQ / 2) * j];
+      }
+    }
+    __syncthreads();  // the previous unit is done with LDS (and the twiddles are in)
     if (!valid) {
       for (int pol = pol0; pol < pol0 + G::PPU; pol++) {
         const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;
@@ -187,10 +211,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR) fstage_mfma_kernel(FParams p, lon
     fill_tables<L, false>(p, frac, a, b, stab, tid, G::NTHR);
     __syncthreads();
 
-    // The polarisations in turn.  Pass 0 of the second writes buffer A,
-    // which every wave finished reading before the barrier after pass 1 of
-    // the first; its pass 1 writes buffer B after the barrier that follows
-    // its pass 0, by which every wave is done with the first's last pass.
+    // The polarisations in turn, through the one buffer.
 #pragma unroll 1
     for (int pol = pol0; pol < pol0 + G::PPU; pol++) {
     const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;
@@ -198,74 +219,102 @@ __global__ void __launch_bounds__(MG<L>::NTHR) fstage_mfma_kernel(FParams p, lon
     // Pass 0: DFT-16 over window samples n = b + LQ r, rotated on the way in
     // (rotator sub[n % S2] * step[n / S2], fxkernel.cpp:553-558; S2 divides
     // LQ, so sub is one entry per column).
-    gbyte *src = (gbyte *)p.station[st];
-    uint32_t *sa = sd, *sb = sd + G::PS;
 #pragma unroll
     for (int k = 0; k < K01; k++) {
       const int bc = 16 * (wave + NW * k) + c16;
-      const int s0 = offset + bc;
-      const int sh = (s0 & 1) * 4 + 2 * pol;
-      gbyte *bp = src + (s0 >> 1) + (LQ / 2) * 4 * g;
+      const int sh = ((offset + bc) & 1) * 4 + 2 * pol;
       const float2 rsub = stab[bc % S2];
       const float2 *stp = stab + S2 + bc / S2 + (LQ / S2) * 4 * g;
       uint32_t w[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const float lev = slev[(bp[(LQ / 2) * j] >> sh) & 3];
+        const float lev = slev[(raw[k][j] >> sh) & 3];
         const float2 r = cmul(rsub, stp[(LQ / S2) * j]);
         w[j] = hbits(toh2(lev * r.x, lev * r.y));
       }
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
-      uint32_t o[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) o[i] = twmul(yr[i], yi[i], tw0[k][i]);
-      *reinterpret_cast<uint4 *>(&sa[pad(16 * bc + 4 * g)]) = make_uint4(o[0], o[1], o[2], o[3]);
+      const uint4 tw = *reinterpret_cast<const uint4 *>(&st0[16 * (bc / RL) + 4 * g]);
+      *reinterpret_cast<uint4 *>(&sd[p0w + 288 * NW * k]) =
+          make_uint4(twmul(yr[0], yi[0], tw.x), twmul(yr[1], yi[1], tw.y), twmul(yr[2], yi[2], tw.z),
+                     twmul(yr[3], yi[3], tw.w));
     }
     __syncthreads();
 
+    // The output scales of the last pass, requested now to arrive during pass 1.
+    float sc[K2][4];
+#pragma unroll
+    for (int k = 0; k < K2; k++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) sc[k][i] = bin_scale<OUT, NCHAN>(p, 2 * st + pol, qb0 + 16 * S * NW * k + 256 * i);
+
     // Pass 1: DFT-16 over in[b + LQ r]; output q1 = (b/16) 256 + b%16 + 16 r'.
+    // Every wave reads its inputs before any writes its outputs.
+    uint32_t o1[K01][4];
 #pragma unroll
     for (int k = 0; k < K01; k++) {
-      const int bc = 16 * (wave + NW * k) + c16;
       uint32_t w[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) w[j] = sa[pad(bc + LQ * (4 * g + j))];
+      for (int j = 0; j < 4; j++) w[j] = sd[p1r + 18 * NW * k + (9 * LQ / 8) * j];
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int q1 = (bc >> 4) * 256 + (bc & 15) + 16 * (4 * g + i);
-        sb[pad(q1)] = twmul(yr[i], yi[i], tw1[k][i]);
-      }
+      const uint4 tw = *reinterpret_cast<const uint4 *>(&st1[256 * (wave + NW * k) + 16 * c16 + 4 * g]);
+      o1[k][0] = twmul(yr[0], yi[0], tw.x);
+      o1[k][1] = twmul(yr[1], yi[1], tw.y);
+      o1[k][2] = twmul(yr[2], yi[2], tw.z);
+      o1[k][3] = twmul(yr[3], yi[3], tw.w);
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K01; k++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) sd[p1w + 288 * NW * k + 16 * i + 4 * (i >> 1)] = o1[k][i];
     __syncthreads();
 
     // Last pass: pruned DFT-RL over in[b + 256 r], b = S C + s for MFMA
     // column C; bin q = b + 256 r' for r' < RL/2, fractional-sample rotation
-    // fsub[q % S] * fstep[q / S] (fxkernel.cpp:614-625) in FP32, store.
+    // fsub[q % TS] * fstep[q / TS] (fxkernel.cpp:614-625) in FP32, store.
+    // TS divides 256 and 16 S NW, so fsub's entry is the lane's for every
+    // tile and row.
+    const float2 fs = fsub[qb0 % TS];
 #pragma unroll
     for (int k = 0; k < K2; k++) {
-      const int C = 16 * (wave + NW * k) + c16;
-      floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+      const int b2 = p2r + 18 * S * NW * k;
+      uint4 x[2];  // B operands of K-steps 0 and 1
 #pragma unroll
       for (int ks = 0; ks < 2; ks++) {
-        uint32_t w[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int kc = 16 * ks + 4 * g + j;
-          w[j] = sb[pad(S * C + kc / RL + 256 * (kc % RL))];
-        }
-        dft_tile(l_rh[ks], l_rl[ks], l_ih[ks], l_il[ks], make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
+        const int o = (16 * ks) / RL + 288 * ((16 * ks) % RL);
+        x[ks] = make_uint4(sd[b2 + o], sd[b2 + o + 288], sd[b2 + o + 576], sd[b2 + o + 864]);
       }
+      // output row 4 g + i sits in element i of lane group g
+      floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+      dft_tile(l_rh, l_rl, l_ih, l_il, x[1], yr, yi);
+      if constexpr (RL == 32) {
+        // (-1)^r on the second K-step's product (odd rows: odd i), then add
+        // the first K-step's into it
+        yr[1] = -yr[1];
+        yr[3] = -yr[3];
+        yi[1] = -yi[1];
+        yi[3] = -yi[3];
+        dft_tile(l_rh, l_rl, l_ih, l_il, x[0], yr, yi);
+      } else {
+        // rows < 8 (g < 2) take the first K-step's product
+        floatx4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+        dft_tile(l_rh, l_rl, l_ih, l_il, x[0], zr, zi);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          yr[i] = g < 2 ? zr[i] : yr[i];
+          yi[i] = g < 2 ? zi[i] : yi[i];
+        }
+      }
+      const int qb = qb0 + 16 * S * NW * k;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int rw = 4 * g + i;
-        const int q = S * C + rw / O + 256 * (rw % O);
-        const float2 fr = cmul(fsub[q % TG::S], fstep[q / TG::S]);
-        store_bin<OUT, NCHAN>(p, row, 2 * st + pol, q, cmul(fr, make_float2(yr[i], yi[i])));
+        const float2 fr = cmul(fs, fstep[qb / TS + (256 / TS) * i]);
+        store_bin_sc<OUT, NCHAN>(p, row, qb + 256 * i, cmul(fr, make_float2(yr[i], yi[i])), sc[k][i]);
       }
     }
+    if (pol + 1 < pol0 + G::PPU) __syncthreads();  // the next pass 0 rewrites the buffer
     }  // pol
   }
 }
