@@ -68,16 +68,39 @@ struct MG {
   static constexpr int NW = NTHR / 64;              // waves per workgroup
   static constexpr int K01 = L / 256 / NW;          // 16-column tiles per wave, passes 0 and 1
   static constexpr int K2 = O / NW;                 // 16-column tiles per wave, last pass
-  static constexpr int PS = L + L / 8;              // padded words of the LDS buffer
-  static constexpr int PPU = L >= 8192 ? 1 : 2;     // polarisations per unit
+  // LDS word of point i in the buffer pass 0 writes (layout A) and in the
+  // one pass 1 writes (layout B): i + a1 (i >> s1) + a2 (i >> s2), chosen so
+  // that the 64 lanes of every LDS access of the passes fall in distinct
+  // banks (64 banks of 4 bytes; the b128 stores in quarter-waves), except
+  // the pass-1 stores at L = 4096 (two lanes to a bank).  Terms of layout A
+  // are multiples of 4, keeping each lane's 16-byte store aligned.  Every
+  // access below is a per-lane base plus an offset that is the same for all
+  // lanes and waves, so the offsets are immediates.
+  static constexpr int A1 = 4, SA1 = 6, A2 = 16, SA2 = L >= 8192 ? 11 : 10;
+  static constexpr int B1 = L == 4096 ? 1 : 16, SB1 = L == 4096 ? 4 : 6;
+  static constexpr int B2 = L == 2048 ? 2 : L == 4096 ? 10 : 16, SB2 = L == 4096 ? 7 : 10;
+  static constexpr int lay_a(int i) { return i + A1 * (i >> SA1) + A2 * (i >> SA2); }
+  static constexpr int lay_b(int i) { return i + B1 * (i >> SB1) + B2 * (i >> SB2); }
+  static constexpr int PS = (lay_a(L - 1) > lay_b(L - 1) ? lay_a(L - 1) : lay_b(L - 1)) + 1;  // buffer words
   using TG = TabGeom<L, false>;
   static_assert(L >= 2048 && L <= 8192, "MFMA F-stage covers L = 2048..8192");
   static_assert(K01 >= 1 && K2 >= 1 && K01 * NW * 256 == L && K2 * NW == O, "whole tiles per wave");
 };
 
-// LDS word of complex point i: 4 words of padding per 32, keeping groups of
-// 4 (one 16-byte store) contiguous and aligned.
-__device__ __forceinline__ int pad(int i) { return i + ((i >> 5) << 2); }
+// Points of the LDS accesses of lane (column c, group g) of wave w:
+// pass-0 store of tile k (4 points from here), pass-1 load j of tile k,
+// pass-1 store i of tile k, last-pass load j of K-step ks of tile k.
+template <int L>
+constexpr int pt0w(int w, int c, int g, int k) { return 16 * (16 * (w + MG<L>::NW * k) + c) + 4 * g; }
+template <int L>
+constexpr int pt1r(int w, int c, int g, int k, int j) { return 16 * (w + MG<L>::NW * k) + c + MG<L>::LQ * (4 * g + j); }
+template <int L>
+constexpr int pt1w(int w, int c, int g, int k, int i) { return 256 * (w + MG<L>::NW * k) + c + 64 * g + 16 * i; }
+template <int L>
+constexpr int pt2r(int w, int c, int g, int k, int ks, int j) {
+  constexpr int RL = MG<L>::RL;
+  return MG<L>::S * (16 * (w + MG<L>::NW * k) + c) + (16 * ks + 4 * g + j) / RL + 256 * ((16 * ks + 4 * g + j) % RL);
+}
 
 // W_L^m = exp(-2 pi i m / L) as packed f16
 template <int L>
@@ -130,7 +153,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   using TG = typename G::TG;
   constexpr int NW = G::NW, K01 = G::K01, K2 = G::K2, RL = G::RL, O = G::O, S = G::S, LQ = G::LQ;
   constexpr int NCHAN = TG::NCHAN, S2 = TG::S2, NROT = TG::S2 + TG::NSTEPS, TS = TG::S;
-  __shared__ __align__(16) uint32_t sd[G::PS];  // the transform: complex f16, padded
+  __shared__ __align__(16) uint32_t sd[G::PS];  // the transform: complex f16, layouts A and B
   __shared__ __align__(16) uint32_t st0[256];   // pass-0 twiddles, [s][r]
   __shared__ __align__(16) uint32_t st1[L];     // pass-1 twiddles, [t][c][g][i]
   __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
@@ -160,14 +183,13 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   for (int e = tid; e < L; e += G::NTHR) st1[e] = twh<L>((((e >> 4) & 15) + 16 * (e & 15)) * (e >> 8));
   if (tid < 4) slev[tid] = tid == 0 ? -kHiMag : tid == 1 ? -1.0f : tid == 2 ? 1.0f : kHiMag;
 
-  // LDS words of this lane's first tile.  pad(x + 32 m) = pad(x) + 36 m, so
-  // the other tiles, rows and K-steps sit at constant offsets from these.
-  const int p0w = pad(256 * wave + 16 * c16 + 4 * g);  // pass-0 store
-  const int p1r = pad(16 * wave + c16 + LQ * 4 * g);   // pass-1 loads
-  const int p1w = pad(256 * wave + c16 + 64 * g);      // pass-1 stores
-  // last-pass loads: input kc = 4 g + j (+ 16 ks) of butterfly b of column C
-  // is point S C + kc / RL + 256 (kc % RL)
-  const int p2r = pad(S * (16 * wave + c16) + (4 * g) / RL + 256 * ((4 * g) % RL));
+  // LDS words of this lane's first access of each kind; the others sit at
+  // constant offsets (last-pass input kc = 4 g + j + 16 ks of the butterfly
+  // of column C is point S C + kc / RL + 256 (kc % RL)).
+  const int p0w = G::lay_a(pt0w<L>(wave, c16, g, 0));
+  const int p1r = G::lay_a(pt1r<L>(wave, c16, g, 0, 0));
+  const int p1w = G::lay_b(pt1w<L>(wave, c16, g, 0, 0));
+  const int p2r = G::lay_b(pt2r<L>(wave, c16, g, 0, 0, 0));
   // last-pass bins q = qb + 256 i (+ 16 S NW per tile): row 4 g + i of
   // column C is output r' = (4 g + i) % O of butterfly (4 g + i) / O
   const int qb0 = S * (16 * wave + c16) + (4 * g) / O + 256 * ((4 * g) % O);
@@ -176,16 +198,14 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   const float2 *fstep = fsub + TS;
 
   for (long long u = blockIdx.x; u < nunits; u += gridDim.x) {
-    const int item = (int)(G::PPU == 2 ? u : u >> 1);
-    const int pol0 = G::PPU == 2 ? 0 : (int)(u & 1);
-    const int tl = item / p.nant;
-    const int st = item % p.nant;
+    const int tl = (int)(u / p.nant);
+    const int st = (int)(u % p.nant);
     const int t = p.t0 + tl * p.tstride;
     int offset = 0;
     double frac = 0.0, a = 0.0, b = 0.0;
     // the same for every thread: the branches below are uniform
     const bool valid = split_offset<L>(p, st, t, offset, frac, a, b);
-    if (tid == 0 && pol0 == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
+    if (tid == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
     // This lane's input bytes (both polarisations) for every pass-0 tile:
     // window samples n = b + LQ r, r = 4 g + j, two samples to a byte.  All
     // are requested here so that their latency overlaps the table build.
@@ -201,7 +221,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     }
     __syncthreads();  // the previous unit is done with LDS (and the twiddles are in)
     if (!valid) {
-      for (int pol = pol0; pol < pol0 + G::PPU; pol++) {
+      for (int pol = 0; pol < 2; pol++) {
         const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;
         for (int q = tid; q < NCHAN; q += G::NTHR)
           store_bin<OUT, NCHAN>(p, row, 2 * st + pol, q, make_float2(0.f, 0.f));
@@ -213,7 +233,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
 
     // The polarisations in turn, through the one buffer.
 #pragma unroll 1
-    for (int pol = pol0; pol < pol0 + G::PPU; pol++) {
+    for (int pol = 0; pol < 2; pol++) {
     const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;
 
     // Pass 0: DFT-16 over window samples n = b + LQ r, rotated on the way in
@@ -235,7 +255,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
       const uint4 tw = *reinterpret_cast<const uint4 *>(&st0[16 * (bc / RL) + 4 * g]);
-      *reinterpret_cast<uint4 *>(&sd[p0w + 288 * NW * k]) =
+      *reinterpret_cast<uint4 *>(&sd[p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)))]) =
           make_uint4(twmul(yr[0], yi[0], tw.x), twmul(yr[1], yi[1], tw.y), twmul(yr[2], yi[2], tw.z),
                      twmul(yr[3], yi[3], tw.w));
     }
@@ -255,7 +275,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     for (int k = 0; k < K01; k++) {
       uint32_t w[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) w[j] = sd[p1r + 18 * NW * k + (9 * LQ / 8) * j];
+      for (int j = 0; j < 4; j++) w[j] = sd[p1r + (G::lay_a(pt1r<L>(0, 0, 0, k, j)) - G::lay_a(pt1r<L>(0, 0, 0, 0, 0)))];
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
       const uint4 tw = *reinterpret_cast<const uint4 *>(&st1[256 * (wave + NW * k) + 16 * c16 + 4 * g]);
@@ -268,7 +288,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
 #pragma unroll
     for (int k = 0; k < K01; k++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) sd[p1w + 288 * NW * k + 16 * i + 4 * (i >> 1)] = o1[k][i];
+      for (int i = 0; i < 4; i++) sd[p1w + (G::lay_b(pt1w<L>(0, 0, 0, k, i)) - G::lay_b(pt1w<L>(0, 0, 0, 0, 0)))] = o1[k][i];
     __syncthreads();
 
     // Last pass: pruned DFT-RL over in[b + 256 r], b = S C + s for MFMA
@@ -279,12 +299,14 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     const float2 fs = fsub[qb0 % TS];
 #pragma unroll
     for (int k = 0; k < K2; k++) {
-      const int b2 = p2r + 18 * S * NW * k;
       uint4 x[2];  // B operands of K-steps 0 and 1
 #pragma unroll
       for (int ks = 0; ks < 2; ks++) {
-        const int o = (16 * ks) / RL + 288 * ((16 * ks) % RL);
-        x[ks] = make_uint4(sd[b2 + o], sd[b2 + o + 288], sd[b2 + o + 576], sd[b2 + o + 864]);
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          w[j] = sd[p2r + (G::lay_b(pt2r<L>(0, 0, 0, k, ks, j)) - G::lay_b(pt2r<L>(0, 0, 0, 0, 0, 0)))];
+        x[ks] = make_uint4(w[0], w[1], w[2], w[3]);
       }
       // output row 4 g + i sits in element i of lane group g
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
@@ -314,14 +336,14 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
         store_bin_sc<OUT, NCHAN>(p, row, qb + 256 * i, cmul(fr, make_float2(yr[i], yi[i])), sc[k][i]);
       }
     }
-    if (pol + 1 < pol0 + G::PPU) __syncthreads();  // the next pass 0 rewrites the buffer
+    if (pol == 0) __syncthreads();  // the next pass 0 rewrites the buffer
     }  // pol
   }
 }
 
 template <int L, int OUT>
 hipError_t launch_l(const FParams &p, hipStream_t s) {
-  const long long nunits = (long long)p.nant * p.nt * (MG<L>::PPU == 2 ? 1 : 2);
+  const long long nunits = (long long)p.nant * p.nt;
   if (nunits <= 0) return hipSuccess;
   const void *k = reinterpret_cast<const void *>(&fstage_mfma_kernel<L, OUT>);
   const long long grid = persistent_grid(k, MG<L>::NTHR, 0, nunits);
