@@ -239,25 +239,43 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // Pass 0: DFT-16 over window samples n = b + LQ r, rotated on the way in
     // (rotator sub[n % S2] * step[n / S2], fxkernel.cpp:553-558; S2 divides
     // LQ, so sub is one entry per column).
-#pragma unroll
-    for (int k = 0; k < K01; k++) {
+    // Each tile's LDS reads are issued one tile ahead of its arithmetic.
+    struct In0 {
+      float2 rsub, step[4];
+      float lev[4];
+      uint4 tw;
+    };
+    auto fetch0 = [&](int k) {
+      In0 f;
       const int bc = 16 * (wave + NW * k) + c16;
       const int sh = ((offset + bc) & 1) * 4 + 2 * pol;
-      const float2 rsub = stab[bc % S2];
       const float2 *stp = stab + S2 + bc / S2 + (LQ / S2) * 4 * g;
+      f.rsub = stab[bc % S2];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        f.lev[j] = slev[(raw[k][j] >> sh) & 3];
+        f.step[j] = stp[(LQ / S2) * j];
+      }
+      f.tw = *reinterpret_cast<const uint4 *>(&st0[16 * (bc / RL) + 4 * g]);
+      return f;
+    };
+    In0 f0 = fetch0(0);
+#pragma unroll
+    for (int k = 0; k < K01; k++) {
+      In0 f1;
+      if (k + 1 < K01) f1 = fetch0(k + 1);
       uint32_t w[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const float lev = slev[(raw[k][j] >> sh) & 3];
-        const float2 r = cmul(rsub, stp[(LQ / S2) * j]);
-        w[j] = hbits(toh2(lev * r.x, lev * r.y));
+        const float2 r = cmul(f0.rsub, f0.step[j]);
+        w[j] = hbits(toh2(f0.lev[j] * r.x, f0.lev[j] * r.y));
       }
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
-      const uint4 tw = *reinterpret_cast<const uint4 *>(&st0[16 * (bc / RL) + 4 * g]);
       *reinterpret_cast<uint4 *>(&sd[p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)))]) =
-          make_uint4(twmul(yr[0], yi[0], tw.x), twmul(yr[1], yi[1], tw.y), twmul(yr[2], yi[2], tw.z),
-                     twmul(yr[3], yi[3], tw.w));
+          make_uint4(twmul(yr[0], yi[0], f0.tw.x), twmul(yr[1], yi[1], f0.tw.y), twmul(yr[2], yi[2], f0.tw.z),
+                     twmul(yr[3], yi[3], f0.tw.w));
+      if (k + 1 < K01) f0 = f1;
     }
     __syncthreads();
 
@@ -271,18 +289,29 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // Pass 1: DFT-16 over in[b + LQ r]; output q1 = (b/16) 256 + b%16 + 16 r'.
     // Every wave reads its inputs before any writes its outputs.
     uint32_t o1[K01][4];
-#pragma unroll
-    for (int k = 0; k < K01; k++) {
+    auto fetch1 = [&](int k, uint4 &x, uint4 &tw) {
       uint32_t w[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) w[j] = sd[p1r + (G::lay_a(pt1r<L>(0, 0, 0, k, j)) - G::lay_a(pt1r<L>(0, 0, 0, 0, 0)))];
+      x = make_uint4(w[0], w[1], w[2], w[3]);
+      tw = *reinterpret_cast<const uint4 *>(&st1[256 * (wave + NW * k) + 16 * c16 + 4 * g]);
+    };
+    uint4 x0, tw0;
+    fetch1(0, x0, tw0);
+#pragma unroll
+    for (int k = 0; k < K01; k++) {
+      uint4 x1, tw1;
+      if (k + 1 < K01) fetch1(k + 1, x1, tw1);
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-      dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
-      const uint4 tw = *reinterpret_cast<const uint4 *>(&st1[256 * (wave + NW * k) + 16 * c16 + 4 * g]);
-      o1[k][0] = twmul(yr[0], yi[0], tw.x);
-      o1[k][1] = twmul(yr[1], yi[1], tw.y);
-      o1[k][2] = twmul(yr[2], yi[2], tw.z);
-      o1[k][3] = twmul(yr[3], yi[3], tw.w);
+      dft_tile(d_rh, d_rl, d_ih, d_il, x0, yr, yi);
+      o1[k][0] = twmul(yr[0], yi[0], tw0.x);
+      o1[k][1] = twmul(yr[1], yi[1], tw0.y);
+      o1[k][2] = twmul(yr[2], yi[2], tw0.z);
+      o1[k][3] = twmul(yr[3], yi[3], tw0.w);
+      if (k + 1 < K01) {
+        x0 = x1;
+        tw0 = tw1;
+      }
     }
     __syncthreads();
 #pragma unroll
