@@ -52,10 +52,15 @@ __device__ __forceinline__ floatx4 mma(const uint4 &a, const uint4 &b, floatx4 c
 __device__ __forceinline__ h2 toh2(float re, float im) { return __builtin_convertvector((float2v){re, im}, h2); }
 __device__ __forceinline__ uint32_t hbits(h2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ h2 hval(uint32_t v) { return __builtin_bit_cast(h2, v); }
-// y * w in FP32 with w stored as an f16 pair, rounded to f16 once
+// y * w in FP32 with w stored as an f16 pair, rounded to f16 once.  Each
+// product is an FMA with the f16 operand widened inside it (v_fma_mix), so
+// w is never converted on its own: re = fma(yr, wr, -(yi wi)),
+// im = fma(yr, wi, yi wr), each inner product an fma with +0.
 __device__ __forceinline__ uint32_t twmul(float yr, float yi, uint32_t w) {
-  const float2v t = __builtin_convertvector(hval(w), float2v);
-  return hbits(toh2(yr * t.x - yi * t.y, yr * t.y + yi * t.x));
+  const h2 v = hval(w);
+  const float wr = (float)v.x, wi = (float)v.y;
+  const float a = __builtin_fmaf(yi, wi, 0.0f), b = __builtin_fmaf(yi, wr, 0.0f);
+  return hbits(toh2(__builtin_fmaf(yr, wr, -a), __builtin_fmaf(yr, wi, b)));
 }
 
 template <int L>
