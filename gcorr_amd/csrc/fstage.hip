@@ -31,8 +31,7 @@ int fstage_blocks(int L, bool cplx, int *out) {
 
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, int block, hipStream_t s) {
   if (block == kFBlockMfma)
-    return fstage_mfma_supported(L, cplx) && p.out_mode != OUT_F32 ? launch_fstage_mfma(p, L, s)
-                                                                   : hipErrorInvalidValue;
+    return fstage_mfma_supported(L, cplx, p.out_mode == OUT_F32) ? launch_fstage_mfma(p, L, s) : hipErrorInvalidValue;
   if (L > 8192 || (L >= 2048 && block == L / 32)) return launch_fstage_big(p, L, cplx, s);
   if (block == 0 || block == fs::fblock_for(L, cplx, 2)) return fs::launch_fstage_mult<2>(p, L, cplx, s);
   if (block == fs::fblock_for(L, cplx, 1)) return launch_fstage_half(p, L, cplx, s);

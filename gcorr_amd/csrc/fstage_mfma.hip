@@ -63,6 +63,42 @@ __device__ __forceinline__ uint32_t twmul(float yr, float yi, uint32_t w) {
   return hbits(toh2(__builtin_fmaf(yr, wr, -a), __builtin_fmaf(yr, wi, b)));
 }
 
+// (re, im) as f16 pair hi and, for the FP32 form, the f16 pair of the
+// remainder lo = (re, im) - hi: hi + lo carries 22 significant bits.
+template <bool SPLIT>
+__device__ __forceinline__ void pack(float re, float im, uint32_t &hi, uint32_t &lo) {
+  const h2 h = toh2(re, im);
+  hi = hbits(h);
+  if constexpr (SPLIT) {
+    const float2v b = __builtin_convertvector(h, float2v);
+    lo = hbits(toh2(re - b.x, im - b.y));
+  }
+}
+
+// Twiddle of the passes: an f16 pair (reduced-precision forms) or FP32.
+template <bool SPLIT>
+using Tw = typename std::conditional<SPLIT, float2, uint32_t>::type;
+
+// y * w, packed as above
+template <bool SPLIT>
+__device__ __forceinline__ void twist(float yr, float yi, Tw<SPLIT> w, uint32_t &hi, uint32_t &lo) {
+  if constexpr (SPLIT) {
+    pack<true>(yr * w.x - yi * w.y, yr * w.y + yi * w.x, hi, lo);
+  } else {
+    hi = twmul(yr, yi, w);
+  }
+}
+
+// B operand of one tile: four f16 pairs and, for the FP32 form, their remainders.
+template <bool SPLIT>
+struct Opnd {
+  uint4 h;
+};
+template <>
+struct Opnd<true> {
+  uint4 h, l;
+};
+
 template <int L>
 struct MG {
   static constexpr int RL = L / 256;                // radix of the last pass
@@ -152,15 +188,40 @@ __device__ __forceinline__ void dft_tile(const uint4 &rh, const uint4 &rl, const
   yi = mma(il, b, yi);
 }
 
+// The same with B as hi + lo for the FP32 form (A_lo B_lo, below 2^-22 of
+// the product, is left out).
+template <bool SPLIT>
+__device__ __forceinline__ void dft_op(const uint4 &rh, const uint4 &rl, const uint4 &ih, const uint4 &il,
+                                       const Opnd<SPLIT> &b, floatx4 &yr, floatx4 &yi) {
+  dft_tile(rh, rl, ih, il, b.h, yr, yi);
+  if constexpr (SPLIT) {
+    yr = mma(rh, b.l, yr);
+    yi = mma(ih, b.l, yi);
+  }
+}
+
+// W_N^m = exp(-2 pi i m / N) in FP32
+template <int N>
+__device__ __forceinline__ float2 twf(int m) {
+  float s, c;
+  sincospif(2.0f * (float)(m % N) / (float)N, &s, &c);
+  return make_float2(c, -s);
+}
+
 template <int L, int OUT>
 __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, long long nunits) {
   using G = MG<L>;
   using TG = typename G::TG;
   constexpr int NW = G::NW, K01 = G::K01, K2 = G::K2, RL = G::RL, O = G::O, S = G::S, LQ = G::LQ;
   constexpr int NCHAN = TG::NCHAN, S2 = TG::S2, NROT = TG::S2 + TG::NSTEPS, TS = TG::S;
-  __shared__ __align__(16) uint32_t sd[G::PS];  // the transform: complex f16, layouts A and B
-  __shared__ __align__(16) uint32_t st0[256];   // pass-0 twiddles, [s][r]
-  __shared__ __align__(16) uint32_t st1[L];     // pass-1 twiddles, [t][c][g][i]
+  // The FP32 form keeps every operand as hi + lo f16 pairs (planes 0 and 1
+  // of the buffer) and its twiddles in FP32.
+  constexpr bool SPLIT = OUT == OUT_F32;
+  constexpr int TWW = SPLIT ? 2 : 1;                   // words per twiddle
+  __shared__ __align__(16) uint32_t sd[TWW * G::PS];  // the transform, layouts A and B
+  __shared__ __align__(16) uint32_t st0[TWW * 256];   // pass-0 twiddles, [s][r]
+  __shared__ __align__(16) uint32_t st1[TWW * L];     // pass-1 twiddles, [t][c][g][i]
+  uint32_t *const sl = sd + (SPLIT ? G::PS : 0);      // plane 1 (FP32 form)
   __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
   __shared__ float slev[4];                     // 2-bit levels (fxkernel.cpp:15-28)
 
@@ -184,8 +245,15 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   // W_L^(r RL (b / RL)) = W_256^(r s), s = b / RL < 16: st0[16 s + r].
   // Pass-1 output q1 = 256 t + c + 16 r (tile t, column c, row r = 4 g + i)
   // takes W_L^((c + 16 r) t): st1[256 t + 16 c + r].
-  for (int e = tid; e < 256; e += G::NTHR) st0[e] = twh<256>((e & 15) * (e >> 4));
-  for (int e = tid; e < L; e += G::NTHR) st1[e] = twh<L>((((e >> 4) & 15) + 16 * (e & 15)) * (e >> 8));
+  for (int e = tid; e < 256; e += G::NTHR) {
+    if constexpr (SPLIT) reinterpret_cast<float2 *>(st0)[e] = twf<256>((e & 15) * (e >> 4));
+    else st0[e] = twh<256>((e & 15) * (e >> 4));
+  }
+  for (int e = tid; e < L; e += G::NTHR) {
+    const int m = (((e >> 4) & 15) + 16 * (e & 15)) * (e >> 8);
+    if constexpr (SPLIT) reinterpret_cast<float2 *>(st1)[e] = twf<L>(m);
+    else st1[e] = twh<L>(m);
+  }
   if (tid < 4) slev[tid] = tid == 0 ? -kHiMag : tid == 1 ? -1.0f : tid == 2 ? 1.0f : kHiMag;
 
   // LDS words of this lane's first access of each kind; the others sit at
@@ -241,6 +309,24 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     for (int pol = 0; pol < 2; pol++) {
     const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;
 
+    // four consecutive twiddles of a table
+    auto fetch_tw = [&](const uint32_t *tab, int e, Tw<SPLIT> *tw) {
+      if constexpr (SPLIT) {
+        const float4 a = reinterpret_cast<const float4 *>(tab)[e / 2];
+        const float4 c = reinterpret_cast<const float4 *>(tab)[e / 2 + 1];
+        tw[0] = make_float2(a.x, a.y);
+        tw[1] = make_float2(a.z, a.w);
+        tw[2] = make_float2(c.x, c.y);
+        tw[3] = make_float2(c.z, c.w);
+      } else {
+        const uint4 v = *reinterpret_cast<const uint4 *>(&tab[e]);
+        tw[0] = v.x;
+        tw[1] = v.y;
+        tw[2] = v.z;
+        tw[3] = v.w;
+      }
+    };
+
     // Pass 0: DFT-16 over window samples n = b + LQ r, rotated on the way in
     // (rotator sub[n % S2] * step[n / S2], fxkernel.cpp:553-558; S2 divides
     // LQ, so sub is one entry per column).
@@ -248,7 +334,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     struct In0 {
       float2 rsub, step[4];
       float lev[4];
-      uint4 tw;
+      Tw<SPLIT> tw[4];
     };
     auto fetch0 = [&](int k) {
       In0 f;
@@ -261,7 +347,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
         f.lev[j] = slev[(raw[k][j] >> sh) & 3];
         f.step[j] = stp[(LQ / S2) * j];
       }
-      f.tw = *reinterpret_cast<const uint4 *>(&st0[16 * (bc / RL) + 4 * g]);
+      fetch_tw(st0, 16 * (bc / RL) + 4 * g, f.tw);
       return f;
     };
     In0 f0 = fetch0(0);
@@ -269,17 +355,23 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     for (int k = 0; k < K01; k++) {
       In0 f1;
       if (k + 1 < K01) f1 = fetch0(k + 1);
-      uint32_t w[4];
+      uint32_t wh[4], wl[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const float2 r = cmul(f0.rsub, f0.step[j]);
-        w[j] = hbits(toh2(f0.lev[j] * r.x, f0.lev[j] * r.y));
+        pack<SPLIT>(f0.lev[j] * r.x, f0.lev[j] * r.y, wh[j], wl[j]);
       }
+      Opnd<SPLIT> xo;
+      xo.h = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+      if constexpr (SPLIT) xo.l = make_uint4(wl[0], wl[1], wl[2], wl[3]);
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-      dft_tile(d_rh, d_rl, d_ih, d_il, make_uint4(w[0], w[1], w[2], w[3]), yr, yi);
-      *reinterpret_cast<uint4 *>(&sd[p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)))]) =
-          make_uint4(twmul(yr[0], yi[0], f0.tw.x), twmul(yr[1], yi[1], f0.tw.y), twmul(yr[2], yi[2], f0.tw.z),
-                     twmul(yr[3], yi[3], f0.tw.w));
+      dft_op<SPLIT>(d_rh, d_rl, d_ih, d_il, xo, yr, yi);
+      uint32_t oh[4], ol[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], f0.tw[i], oh[i], ol[i]);
+      const int a0 = p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)));
+      *reinterpret_cast<uint4 *>(&sd[a0]) = make_uint4(oh[0], oh[1], oh[2], oh[3]);
+      if constexpr (SPLIT) *reinterpret_cast<uint4 *>(&sl[a0]) = make_uint4(ol[0], ol[1], ol[2], ol[3]);
       if (k + 1 < K01) f0 = f1;
     }
     __syncthreads();
@@ -293,36 +385,46 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
 
     // Pass 1: DFT-16 over in[b + LQ r]; output q1 = (b/16) 256 + b%16 + 16 r'.
     // Every wave reads its inputs before any writes its outputs.
-    uint32_t o1[K01][4];
-    auto fetch1 = [&](int k, uint4 &x, uint4 &tw) {
-      uint32_t w[4];
+    uint32_t o1h[K01][4], o1l[K01][4];
+    auto fetch1 = [&](int k, Opnd<SPLIT> &x, Tw<SPLIT> *tw) {
+      uint32_t wh[4], wl[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) w[j] = sd[p1r + (G::lay_a(pt1r<L>(0, 0, 0, k, j)) - G::lay_a(pt1r<L>(0, 0, 0, 0, 0)))];
-      x = make_uint4(w[0], w[1], w[2], w[3]);
-      tw = *reinterpret_cast<const uint4 *>(&st1[256 * (wave + NW * k) + 16 * c16 + 4 * g]);
+      for (int j = 0; j < 4; j++) {
+        const int a = p1r + (G::lay_a(pt1r<L>(0, 0, 0, k, j)) - G::lay_a(pt1r<L>(0, 0, 0, 0, 0)));
+        wh[j] = sd[a];
+        if constexpr (SPLIT) wl[j] = sl[a];
+      }
+      x.h = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+      if constexpr (SPLIT) x.l = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+      fetch_tw(st1, 256 * (wave + NW * k) + 16 * c16 + 4 * g, tw);
     };
-    uint4 x0, tw0;
+    Opnd<SPLIT> x0;
+    Tw<SPLIT> tw0[4];
     fetch1(0, x0, tw0);
 #pragma unroll
     for (int k = 0; k < K01; k++) {
-      uint4 x1, tw1;
+      Opnd<SPLIT> x1;
+      Tw<SPLIT> tw1[4];
       if (k + 1 < K01) fetch1(k + 1, x1, tw1);
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-      dft_tile(d_rh, d_rl, d_ih, d_il, x0, yr, yi);
-      o1[k][0] = twmul(yr[0], yi[0], tw0.x);
-      o1[k][1] = twmul(yr[1], yi[1], tw0.y);
-      o1[k][2] = twmul(yr[2], yi[2], tw0.z);
-      o1[k][3] = twmul(yr[3], yi[3], tw0.w);
+      dft_op<SPLIT>(d_rh, d_rl, d_ih, d_il, x0, yr, yi);
+#pragma unroll
+      for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], tw0[i], o1h[k][i], o1l[k][i]);
       if (k + 1 < K01) {
         x0 = x1;
-        tw0 = tw1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) tw0[i] = tw1[i];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K01; k++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) sd[p1w + (G::lay_b(pt1w<L>(0, 0, 0, k, i)) - G::lay_b(pt1w<L>(0, 0, 0, 0, 0)))] = o1[k][i];
+      for (int i = 0; i < 4; i++) {
+        const int a = p1w + (G::lay_b(pt1w<L>(0, 0, 0, k, i)) - G::lay_b(pt1w<L>(0, 0, 0, 0, 0)));
+        sd[a] = o1h[k][i];
+        if constexpr (SPLIT) sl[a] = o1l[k][i];
+      }
     __syncthreads();
 
     // Last pass: pruned DFT-RL over in[b + 256 r], b = S C + s for MFMA
@@ -333,18 +435,22 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     const float2 fs = fsub[qb0 % TS];
 #pragma unroll
     for (int k = 0; k < K2; k++) {
-      uint4 x[2];  // B operands of K-steps 0 and 1
+      Opnd<SPLIT> x[2];  // B operands of K-steps 0 and 1
 #pragma unroll
       for (int ks = 0; ks < 2; ks++) {
-        uint32_t w[4];
+        uint32_t wh[4], wl[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-          w[j] = sd[p2r + (G::lay_b(pt2r<L>(0, 0, 0, k, ks, j)) - G::lay_b(pt2r<L>(0, 0, 0, 0, 0, 0)))];
-        x[ks] = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int j = 0; j < 4; j++) {
+          const int a = p2r + (G::lay_b(pt2r<L>(0, 0, 0, k, ks, j)) - G::lay_b(pt2r<L>(0, 0, 0, 0, 0, 0)));
+          wh[j] = sd[a];
+          if constexpr (SPLIT) wl[j] = sl[a];
+        }
+        x[ks].h = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+        if constexpr (SPLIT) x[ks].l = make_uint4(wl[0], wl[1], wl[2], wl[3]);
       }
       // output row 4 g + i sits in element i of lane group g
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-      dft_tile(l_rh, l_rl, l_ih, l_il, x[1], yr, yi);
+      dft_op<SPLIT>(l_rh, l_rl, l_ih, l_il, x[1], yr, yi);
       if constexpr (RL == 32) {
         // (-1)^r on the second K-step's product (odd rows: odd i), then add
         // the first K-step's into it
@@ -352,11 +458,11 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
         yr[3] = -yr[3];
         yi[1] = -yi[1];
         yi[3] = -yi[3];
-        dft_tile(l_rh, l_rl, l_ih, l_il, x[0], yr, yi);
+        dft_op<SPLIT>(l_rh, l_rl, l_ih, l_il, x[0], yr, yi);
       } else {
         // rows < 8 (g < 2) take the first K-step's product
         floatx4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
-        dft_tile(l_rh, l_rl, l_ih, l_il, x[0], zr, zi);
+        dft_op<SPLIT>(l_rh, l_rl, l_ih, l_il, x[0], zr, zi);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           yr[i] = g < 2 ? zr[i] : yr[i];
@@ -388,6 +494,9 @@ hipError_t launch_l(const FParams &p, hipStream_t s) {
 template <int L>
 hipError_t launch_out(const FParams &p, hipStream_t s) {
   switch (p.out_mode) {
+    case OUT_F32:
+      if constexpr (L <= 4096) return launch_l<L, OUT_F32>(p, s);
+      return hipErrorInvalidValue;
     case OUT_F16: return launch_l<L, OUT_F16>(p, s);
     case OUT_BF16: return launch_l<L, OUT_BF16>(p, s);
     case OUT_FP8: return launch_l<L, OUT_FP8>(p, s);
@@ -399,7 +508,7 @@ hipError_t launch_out(const FParams &p, hipStream_t s) {
 }  // namespace
 }  // namespace fs
 
-bool fstage_mfma_supported(int L, bool cplx) { return !cplx && L >= 2048 && L <= 8192; }
+bool fstage_mfma_supported(int L, bool cplx, bool fp32) { return !cplx && L >= 2048 && L <= (fp32 ? 4096 : 8192); }
 
 hipError_t launch_fstage_mfma(const FParams &p, int L, hipStream_t s) {
   switch (L) {

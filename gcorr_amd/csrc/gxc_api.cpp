@@ -390,8 +390,8 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   {
     int fb[4], nfb = gxc::fstage_blocks(d.L, cfg->is_complex != 0, fb);
     bool ok = cfg->fblock == 0 ||
-              (cfg->fblock == gxc::kFBlockMfma && out_mode != gxc::OUT_F32 &&
-               gxc::fstage_mfma_supported(d.L, cfg->is_complex != 0));
+              (cfg->fblock == gxc::kFBlockMfma &&
+               gxc::fstage_mfma_supported(d.L, cfg->is_complex != 0, out_mode == gxc::OUT_F32));
     for (int i = 0; i < nfb; i++) ok = ok || cfg->fblock == fb[i];
     if (!ok) return fail(set_err(h, GXC_ERR_INVALID_ARG, "fblock is not an F-stage workgroup size for this FFT length"));
     int xt[3], nxt = gxc::xm_tiles(d.M, out_mode, xt);

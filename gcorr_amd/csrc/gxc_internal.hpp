@@ -94,9 +94,10 @@ int fstage_blocks(int L, bool cplx, int *out /* [4] */);
 hipError_t launch_fstage(const FParams &p, int L, bool cplx, int block, hipStream_t s);
 hipError_t launch_fstage_big(const FParams &p, int L, bool cplx, hipStream_t s);  // L = 16384, 32768
 // F-stage with the FFT passes on the matrix cores (fstage_mfma.hip): real
-// input, L = 2048..8192, reduced-precision outputs (and the power pass).
+// input; reduced-precision outputs (and the power pass) at L = 2048..8192,
+// FP32 spectra (operands as f16 hi + lo) at L = 2048..4096.
 constexpr int kFBlockMfma = -1;
-bool fstage_mfma_supported(int L, bool cplx);
+bool fstage_mfma_supported(int L, bool cplx, bool fp32);
 hipError_t launch_fstage_mfma(const FParams &p, int L, hipStream_t s);
 hipError_t launch_counts(const CountParams &p, hipStream_t s);
 // Multi-subband F-stage: real 2-bit input, FFT lengths 256..8192, workgroups

@@ -68,7 +68,7 @@ gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out) {
   double best_f = 1e30;
   int best_fb = h->fblock;
   int fb[5], nfb = gxc::fstage_blocks(h->d.L, h->cfg.is_complex != 0, fb);
-  if (h->out_mode != gxc::OUT_F32 && gxc::fstage_mfma_supported(h->d.L, h->cfg.is_complex != 0))
+  if (gxc::fstage_mfma_supported(h->d.L, h->cfg.is_complex != 0, h->out_mode == gxc::OUT_F32))
     fb[nfb++] = gxc::kFBlockMfma;
   for (int i = 0; i < nfb && st == GXC_OK; i++) {
     h->fblock = fb[i];

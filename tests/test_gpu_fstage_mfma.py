@@ -1,11 +1,12 @@
-"""The matrix-core F-stage (gxc_config.fblock = GXC_FBLOCK_MFMA) for the
-reduced-precision forms.
+"""The matrix-core F-stage (gxc_config.fblock = GXC_FBLOCK_MFMA).
 
-Its FFT passes run as f16 MFMA products with FP32 accumulation, so its
-spectra differ from the FP32 F-stage by f16 roundings of the data between
-passes; the visibilities must stay within the reduced-precision bound of the
-FP64 oracle (1e-3 for FP16, as the FP32-F-stage form), for every supported
-length, with the row counts exact.
+Its FFT passes run as f16 MFMA products with FP32 accumulation.  For the
+reduced-precision forms the data are rounded to f16 between passes, so the
+visibilities must stay within the reduced-precision bound of the FP64 oracle
+(1e-3 for FP16, as the FP32-F-stage form).  The FP32 form carries every
+operand as an f16 pair plus its remainder (22 significant bits) and must meet
+the FP32 bound, 1e-5.  Every supported length is covered, with the row
+counts exact.
 """
 import numpy as np
 import pytest
@@ -82,8 +83,37 @@ def test_mfma_fstage_random_bytes_and_invalid_ffts(gpu):
     assert eps(v, ref) < 1e-3
 
 
+@pytest.mark.parametrize("nchan,numffts", [(1024, 300), (2048, 200)])
+def test_mfma_fstage_fp32_within_bound(gpu, nchan, numffts):
+    w, data, poly, offs = case(4, nchan, numffts, seed=64)
+    ref, rc = fxref.process(fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True),
+                            data, poly, offs, "f64")
+    v, c = run(w, data, poly, offs, precision="fp32", fblock=MFMA)
+    np.testing.assert_array_equal(c, rc)
+    e = eps(v, ref)
+    v0, _ = run(w, data, poly, offs, precision="fp32")
+    e0 = eps(v0, ref)
+    print("nchan %d: FP32 mfma F-stage eps %.3g, FP32 F-stage eps %.3g" % (nchan, e, e0))
+    assert e < 1e-5
+
+
+def test_mfma_fstage_fp32_invalid_ffts(gpu):
+    w = synth.Workload("r", nant=3, nchan=1024, numffts=120, bw=64e6, lo=1.65e9, seed=65)
+    rng = np.random.default_rng(65)
+    data = [rng.integers(0, 256, w.station_bytes, dtype=np.uint8) for _ in range(3)]
+    poly, offs = synth.make_delays(w)
+    offs = offs.copy()
+    offs[2] += 7 * w.fftchannels * w.sampletime
+    ref, rc = fxref.process(fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True),
+                            data, poly, offs, "f64")
+    v, c = run(w, data, poly, offs, precision="fp32", fblock=MFMA)
+    np.testing.assert_array_equal(c, rc)
+    assert c.min() < w.numffts
+    assert eps(v, ref) < 1e-5
+
+
 def test_mfma_fstage_rejected_where_unsupported(gpu):
-    with pytest.raises(gcorr_amd.GxcError):  # FP32 keeps the FP32 F-stage
-        gcorr_amd.FxKernel(3, 1024, 10, 2, 1.65e9, 64e6, precision="fp32", fblock=MFMA)
+    with pytest.raises(gcorr_amd.GxcError):  # FP32 form: up to 4096 points
+        gcorr_amd.FxKernel(3, 4096, 10, 2, 1.65e9, 64e6, precision="fp32", fblock=MFMA)
     with pytest.raises(gcorr_amd.GxcError):  # 512-point FFT: not covered
         gcorr_amd.FxKernel(3, 256, 10, 2, 1.65e9, 64e6, precision="fp16", fblock=MFMA)
