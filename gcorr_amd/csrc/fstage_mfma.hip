@@ -1,5 +1,6 @@
-// F-stage on the matrix cores for the reduced-precision forms (FP16, BF16
-// and FP8 spectra), real 2-bit input, FFT lengths 2048..8192.
+// F-stage on the matrix cores: real 2-bit input; FP16, BF16 and FP8 spectra
+// (and the power pass) at FFT lengths 2048..8192, FP32 spectra at
+// 2048..4096.
 //
 // The FP32 F-stage issues about one VALU wave-instruction per sample and is
 // bound by that.  Here the radix passes run as MFMA products: a radix-R
@@ -13,21 +14,23 @@
 // bins below nchan (fxkernel keeps bins [0, nchan) of the 2*nchan-point FFT),
 // 16/(RL/2) butterflies per MFMA column through a block-diagonal matrix.
 //
-// The data enter every MFMA as f16 and leave it as FP32.  Between passes
-// the FP32 outputs are multiplied by the next pass's twiddle in FP32 and
-// rounded to f16 once; the twiddles of a lane's points do not depend on the
-// unit, so each lane keeps them in registers (as f16 pairs, widened exactly
-// in the multiply) for the whole launch.  The fringe rotation is FP32 with
-// one rounding of the rotated sample.  The last pass's outputs keep FP32 for
-// the fractional-sample rotation and the scaled store.
+// For the reduced-precision forms the data enter every MFMA as f16 and
+// leave it as FP32.  Between passes the FP32 outputs are multiplied by the
+// next pass's twiddle (an f16 pair from an LDS table, widened inside the
+// FMAs) and rounded to f16 once.  The FP32 form carries each value as an f16
+// pair plus the f16 pair of its remainder (22 significant bits), multiplies
+// both into the accumulators (A_hi B_hi + A_lo B_hi + A_hi B_lo) and keeps
+// its twiddles in FP32.  The fringe rotation is FP32 with one rounding of
+// the rotated sample; the last pass's outputs keep FP32 for the
+// fractional-sample rotation and the store.
 //
-// A workgroup of 256 threads (512 at L = 8192) loops over units
-// (persistent grid).  Below L = 8192 a unit is one (station, FFT): the
-// rotator and fractional-sample tables are built once and serve both
-// polarisations, which run one after the other.  At L = 8192 a unit is one
-// polarisation: both in one unit took 25.1 ms for the F-stage of 16
-// stations, 4096 channels, 1 s, one per unit 15.4 ms.  The passes go
-// through two LDS buffers one 16-column tile at a time.
+// A workgroup of 256 threads (512 at L = 8192) loops over units (persistent
+// grid); a unit is one (station, FFT), whose rotator and fractional-sample
+// tables serve both polarisations, run one after the other through one LDS
+// buffer.  Registers are held to 128 per lane so that 4 waves share each
+// SIMD: the twiddles live in LDS tables, every LDS access is a per-lane base
+// plus an immediate offset, and the input bytes and output scales are
+// requested ahead of their use.
 //
 // Everything else follows the FP32 F-stage: the FP64 delay split and the
 // rotator and fractional-sample tables (fill_tables), the 2-bit levels of

@@ -94,8 +94,9 @@ typedef struct {
 } gxc_config;
 
 /* gxc_config.fblock value selecting the F-stage whose FFT passes run on the
- * matrix cores (f16 operands, FP32 accumulation): reduced precision only,
- * real input, fftchannels 2048..8192. */
+ * matrix cores (f16 operands, FP32 accumulation): real input; fftchannels
+ * 2048..8192 for the reduced precisions, 2048..4096 for FP32 (operands as
+ * f16 pairs plus remainders, within the FP32 bound). */
 #define GXC_FBLOCK_MFMA (-1)
 
 typedef struct gxc_handle gxc_handle;
