@@ -96,7 +96,8 @@ class BatchConfig(ctypes.Structure):
         ("is_complex", ctypes.c_int), ("precision", ctypes.c_int), ("autos", ctypes.c_int),
         ("exact_phase", ctypes.c_int), ("nsub", ctypes.c_int),
         ("subbands", ctypes.POINTER(Subband)), ("chunks", ctypes.c_int),
-        ("overlap", ctypes.c_int), ("graph", ctypes.c_int), ("reserved", ctypes.c_int * 4),
+        ("overlap", ctypes.c_int), ("graph", ctypes.c_int), ("fstage_mfma", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 

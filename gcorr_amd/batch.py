@@ -19,7 +19,7 @@ class Batch:
     """Batch(nant, [(nchan, numffts, lo, bw), ...])."""
 
     def __init__(self, nant, subbands, *, precision="fp32", autos=False, exact_phase=False,
-                 chunks=0, overlap=False, graph=True):
+                 chunks=0, overlap=False, graph=True, fstage_mfma=False):
         self._L = _lib.load()
         c = _lib.BatchConfig()
         self._L.gxc_batch_config_init(ctypes.byref(c))
@@ -35,6 +35,7 @@ class Batch:
         c.chunks = int(chunks)
         c.overlap = int(bool(overlap))
         c.graph = int(bool(graph))
+        c.fstage_mfma = int(bool(fstage_mfma))
         self.nant = int(nant)
         self.nrows = num_rows(nant, autos)
         h = ctypes.c_void_p()

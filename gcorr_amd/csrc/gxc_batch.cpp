@@ -69,7 +69,9 @@ struct gxc_batch {
   int2 *d_rowmap = nullptr;
   void *d_spec[2] = {nullptr, nullptr};
   // segment tables and unit totals
-  gxc::FSeg *d_fseg = nullptr;  // [nchunks][nsub]
+  gxc::FSeg *d_fseg = nullptr;  // [nchunks][nmulti]: subbands of the multi-subband F launch
+  int nmulti = 0;
+  std::vector<std::vector<gxc::FSeg>> mseg;  // [nchunks]: subbands on the matrix-core F-stage
   gxc::XSeg *d_xseg = nullptr;  // [nchunks][nsub]
   gxc::FSeg *d_powseg = nullptr;
   gxc::ScaleSeg *d_scaleseg = nullptr;
@@ -184,22 +186,35 @@ gxc_status build_tables(gxc_batch *b) {
   b->d_scaleseg = nullptr;
   const int ns = (int)b->sub.size(), C = b->nchunks;
   const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
-  std::vector<gxc::FSeg> fseg((size_t)C * ns);
+  auto on_mfma = [&](const Sub &u) {
+    return b->cfg.fstage_mfma && gxc::fstage_mfma_supported(u.d.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32);
+  };
+  b->nmulti = 0;
+  for (const Sub &u : b->sub) b->nmulti += on_mfma(u) ? 0 : 1;
+  b->mseg.assign(C, {});
+  std::vector<gxc::FSeg> fseg((size_t)C * std::max(1, b->nmulti));
   std::vector<gxc::XSeg> xseg((size_t)C * ns);
   b->ftotal.assign(C, 0);
   b->xtotal.assign(C, 0);
   for (int c = 0; c < C; c++) {
     const int buf = b->nbuf == 2 ? (c & 1) : 0;
     long long fu = 0, xu = 0;
+    int m = 0;
     for (int s = 0; s < ns; s++) {
       const Sub &u = b->sub[s];
       const int t0 = chunk_begin(u, c, C), nt = chunk_begin(u, c + 1, C) - t0;
       void *spec = static_cast<unsigned char *>(b->d_spec[buf]) + u.spec_off;
-      gxc::FSeg &f = fseg[(size_t)c * ns + s];
+      gxc::FSeg f;
       f.p = fparams(b, u, t0, nt, 1, b->out_mode, spec);
       f.L = u.d.L;
-      f.unit0 = fu;
-      fu += gxc::fstage_multi_units(u.d.L, b->fblock, (long long)b->cfg.nant * nt);
+      if (on_mfma(u)) {
+        f.unit0 = 0;
+        b->mseg[c].push_back(f);
+      } else {
+        f.unit0 = fu;
+        fu += gxc::fstage_multi_units(u.d.L, b->fblock, (long long)b->cfg.nant * nt);
+        fseg[(size_t)c * b->nmulti + m++] = f;
+      }
       gxc::XSeg &x = xseg[(size_t)c * ns + s];
       x.p.spectra = spec;
       x.p.partial = u.d_partial;
@@ -314,7 +329,10 @@ gxc_status enqueue_device_work(gxc_batch *b) {
     const int buf = overlap ? (c & 1) : 0;
     if (overlap && c >= 2) GXB_HIP(b, hipStreamWaitEvent(fs, b->ev_x[buf], 0));
     mark(b, 0, fs);
-    GXB_HIP(b, gxc::launch_fstage_multi(b->d_fseg + (size_t)c * ns, ns, b->ftotal[c], b->out_mode, b->fblock, fs));
+    if (b->nmulti > 0 && b->ftotal[c] > 0)
+      GXB_HIP(b, gxc::launch_fstage_multi(b->d_fseg + (size_t)c * b->nmulti, b->nmulti, b->ftotal[c], b->out_mode,
+                                          b->fblock, fs));
+    for (const gxc::FSeg &f : b->mseg[c]) GXB_HIP(b, gxc::launch_fstage_mfma(f.p, f.L, fs));
     mark(b, 1, fs);
     if (overlap) {
       GXB_HIP(b, hipEventRecord(b->ev_f[buf], fs));
@@ -726,22 +744,29 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
   double def_ms = 0, best = 1e30;
   gxc_status st = time_process(b, &def_ms);
   cand++;
-  int best_fb = b->fblock, best_units = b->xunits;
-  for (int fb : {256, 512}) {
-    if (st != GXC_OK) continue;
-    for (int units : {1024, 2048, 4096, 8192}) {
-      b->fblock = fb;
-      b->xunits = units;
-      st = assign_splits(b);
-      if (st == GXC_OK) st = build_tables(b);
-      double ms = 0;
-      if (st == GXC_OK) st = time_process(b, &ms);
-      cand++;
-      if (st != GXC_OK) break;
-      if (ms < best) best = ms, best_fb = fb, best_units = units;
+  int best_fb = b->fblock, best_units = b->xunits, best_mfma = b->cfg.fstage_mfma;
+  bool any_mfma = false;
+  for (const Sub &u : b->sub)
+    any_mfma = any_mfma ||
+               gxc::fstage_mfma_supported(u.d.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32);
+  for (int mf = 0; mf <= (any_mfma ? 1 : 0); mf++)
+    for (int fb : {256, 512}) {
+      if (st != GXC_OK) continue;
+      for (int units : {1024, 2048, 4096, 8192}) {
+        b->cfg.fstage_mfma = mf;
+        b->fblock = fb;
+        b->xunits = units;
+        st = assign_splits(b);
+        if (st == GXC_OK) st = build_tables(b);
+        double ms = 0;
+        if (st == GXC_OK) st = time_process(b, &ms);
+        cand++;
+        if (st != GXC_OK) break;
+        if (ms < best) best = ms, best_fb = fb, best_units = units, best_mfma = mf;
+      }
     }
-  }
   if (st == GXC_OK) {
+    b->cfg.fstage_mfma = best_mfma;
     b->fblock = best_fb;
     b->xunits = best_units;
     st = assign_splits(b);
@@ -758,7 +783,7 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
   b->graph_exec = nullptr;
   b->profiling = prof;
   if (st != GXC_OK) return st;
-  t.fblock = b->fblock;
+  t.fblock = b->cfg.fstage_mfma ? gxc::kFBlockMfma : b->fblock;
   t.xtile = gxc::xm_channels_per_group(b->M, b->out_mode);
   t.xsplits = b->xunits;  // for a batch: the X units per launch the splits aim at
   t.graph = b->cfg.graph;

@@ -79,6 +79,40 @@ def test_batch_fp16_vs_fp64_oracle(gpu):
         assert e < 1e-3, s
 
 
+@pytest.mark.parametrize("precision,bound,mode", [("fp32", EPS, "f32"), ("fp16", 1e-3, "f64")])
+def test_batch_matrix_core_fstage(gpu, precision, bound, mode):
+    """fstage_mfma: the 2048..8192-point subbands (2048..4096 for FP32) on the
+    matrix-core F-stage, one launch each, the others in the multi launch."""
+    ws = subbands(div=125)
+    gen = [synth.generate(w, device="cpu") for w in ws]
+    datas = [[b.numpy() for b in g[0]] for g in gen]
+    delays = [(g[1], g[2]) for g in gen]
+    got = run_batch(ws, datas, delays, precision=precision, fstage_mfma=True, chunks=2)
+    for s, w in enumerate(ws):
+        v, c = got[s]
+        ref, rc = oracle(w, datas[s], *delays[s], mode=mode)
+        np.testing.assert_array_equal(c, rc)
+        e = eps(v, ref)
+        print(w.name, precision, "matrix-core batch eps", e)
+        assert e < bound, s
+
+
+def test_batch_matrix_core_modes_agree(gpu):
+    ws = subbands(div=500)
+    datas = random_inputs(ws, seed=7)
+    delays = [synth.make_delays(w) for w in ws]
+    base = None
+    for overlap in (False, True):
+        for graph in (False, True):
+            got = run_batch(ws, datas, delays, precision="fp16", fstage_mfma=True, chunks=4, overlap=overlap,
+                            graph=graph)
+            if base is None:
+                base = got
+            for (v, c), (v0, c0) in zip(got, base):
+                np.testing.assert_array_equal(c, c0)
+                np.testing.assert_array_equal(v, v0)
+
+
 def test_batch_modes_agree(gpu):
     """Overlap and graph replay change scheduling only: identical results."""
     ws = subbands(div=500)
@@ -129,7 +163,7 @@ def test_batch_autotune_keeps_results(gpu):
         b.set_input(s, datas[s])
         b.set_delays(s, *delays[s])
     t = b.autotune()
-    assert t["candidates"] >= 5 and t["fblock"] in (256, 512) and t["process_ms"] > 0
+    assert t["candidates"] >= 5 and t["fblock"] in (256, 512, -1) and t["process_ms"] > 0
     b.process()
     for s, w in enumerate(ws):
         ref, rc = oracle(w, datas[s], *delays[s])
