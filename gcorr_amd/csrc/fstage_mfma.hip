@@ -102,6 +102,13 @@ struct Opnd<true> {
   uint4 h, l;
 };
 
+// Delay split of one unit (split_offset's results).
+struct Split {
+  double frac, a, b;
+  int offset, valid;
+};
+constexpr int kSplitBatch = 16;  // units whose splits one wave computes at once
+
 template <int L>
 struct MG {
   static constexpr int RL = L / 256;                // radix of the last pass
@@ -227,6 +234,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   uint32_t *const sl = sd + (SPLIT ? G::PS : 0);      // plane 1 (FP32 form)
   __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
   __shared__ float slev[4];                     // 2-bit levels (fxkernel.cpp:15-28)
+  __shared__ Split ssplit[kSplitBatch];         // this workgroup's next units' delay splits
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c16 = lane & 15, g = lane >> 4;
@@ -273,14 +281,32 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   const float2 *fsub = stab + NROT;
   const float2 *fstep = fsub + TS;
 
-  for (long long u = blockIdx.x; u < nunits; u += gridDim.x) {
+  int it = 0;
+  for (long long u = blockIdx.x; u < nunits; u += gridDim.x, it++) {
+    // The FP64 delay splits of this workgroup's next kSplitBatch units, one
+    // per lane of wave 0 (instead of every thread redoing its unit's).
+    // Entries are read before the barrier at the top of each unit, so the
+    // batch is only rewritten after every thread has read the last one.
+    if (it % kSplitBatch == 0) {
+      if (tid < kSplitBatch) {
+        const long long v = u + (long long)gridDim.x * tid;
+        Split sp{0.0, 0.0, 0.0, 0, 0};
+        if (v < nunits) {
+          const int vt = p.t0 + (int)(v / p.nant) * p.tstride;
+          sp.valid = split_offset<L>(p, (int)(v % p.nant), vt, sp.offset, sp.frac, sp.a, sp.b) ? 1 : 0;
+        }
+        ssplit[tid] = sp;
+      }
+      __syncthreads();
+    }
     const int tl = (int)(u / p.nant);
     const int st = (int)(u % p.nant);
     const int t = p.t0 + tl * p.tstride;
-    int offset = 0;
-    double frac = 0.0, a = 0.0, b = 0.0;
+    const Split sp = ssplit[it % kSplitBatch];
+    const int offset = sp.offset;
+    const double frac = sp.frac, a = sp.a, b = sp.b;
     // the same for every thread: the branches below are uniform
-    const bool valid = split_offset<L>(p, st, t, offset, frac, a, b);
+    const bool valid = sp.valid != 0;
     if (tid == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
     // This lane's input bytes (both polarisations) for every pass-0 tile:
     // window samples n = b + LQ r, r = 4 g + j, two samples to a byte.  All

@@ -379,7 +379,10 @@ def test_autotune_keeps_parity_and_reports_choice(gpu, precision):
     """The autotuner only changes launch shapes: after it, results still
     match the oracle (FP32), or the untuned handle within FP16 resolution
     (another F-stage kernel rounds the FP32 spectra differently, which moves
-    some FP16 roundings by one unit: 2^-11 relative on single products)."""
+    some FP16 roundings by one unit: 2^-11 relative on single products).
+    When it picks the matrix-core F-stage (fblock -1), whose FFT passes
+    round to f16, the FP16 result is held to the FP16 bound of the FP64
+    oracle instead."""
     w, data, poly, offs = random_case(6, 1024, 400, seed=13)
     k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True,
                            precision=precision)
@@ -388,14 +391,19 @@ def test_autotune_keeps_parity_and_reports_choice(gpu, precision):
     k.process()
     before = k.visibilities()
     t = k.autotune()
-    assert t["candidates"] >= 5 and t["fblock"] > 0 and t["xtile"] > 0 and t["xsplits"] >= 1
+    assert t["candidates"] >= 5 and (t["fblock"] > 0 or t["fblock"] == -1)
+    assert t["xtile"] > 0 and t["xsplits"] >= 1
     assert t["process_ms"] > 0 and t["default_ms"] > 0
     k.process()
     after = k.visibilities()
     if precision == "fp32":
         ref, _ = run_oracle(w, data, poly, offs)
         assert eps(after, ref) < EPS
-    assert eps(after, before) < (1e-5 if precision == "fp32" else 1e-4)
+    if precision == "fp16" and t["fblock"] == -1:
+        ref64, _ = run_oracle(w, data, poly, offs, mode="f64")
+        assert eps(after, ref64) < 1e-3
+    else:
+        assert eps(after, before) < (1e-5 if precision == "fp32" else 1e-4)
     # the choice handed to a new handle through the config gives the same result
     k2 = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True,
                             precision=precision, fblock=t["fblock"], xtile=t["xtile"],
