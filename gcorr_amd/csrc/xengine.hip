@@ -38,7 +38,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // input blocks.
 constexpr int xm_default_ct(int et, int mb) { return (et == OUT_F32 ? 16 : 32) / mb; }
 
-template <int ET, int MB, int CT_ = xm_default_ct(ET, MB)>
+template <int ET, int MB, int CT_ = xm_default_ct(ET, MB), int NWV_ = 4>
 struct XM {
   static constexpr bool F32 = ET == OUT_F32;
   static constexpr int EB = F32 ? 4 : ET == OUT_FP8 ? 1 : 2;  // bytes per real component
@@ -49,7 +49,9 @@ struct XM {
   static constexpr int CT = CT_;                             // channels per workgroup
   static constexpr int NTILE = MB * (MB + 1) / 2;
   static constexpr int EPL = 16 / CB;  // complex values (channels) per 16-byte load
-  static constexpr int CPW = CT / 4;   // channels per wave
+  static constexpr int NWV = NWV_;     // waves per workgroup
+  static constexpr int NTHR = 64 * NWV;
+  static constexpr int CPW = CT / NWV;  // channels per wave
   // 16/8-bit: per plane, rows m of TB*EB bytes = TB/8 chunks of 8 FFTs.
   // FP32: per plane, rows t of ROWB bytes holding the inputs contiguously,
   // padded so that 4 consecutive rows start 64 or 192 bytes apart mod 256.
@@ -100,17 +102,17 @@ struct Operand {
 // it transposed into LDS.  Tasks are strided over the 256 threads; in a
 // task the channel column is the fastest index, so neighbouring lanes load
 // neighbouring 16-byte columns of one spectra row.
-template <int ET, int MB, int CT>
+template <int ET, int MB, int CT, int NWV = 4>
 __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned char *src, size_t rowbytes,
                                            int M, int c0, int ts, int te, int tid) {
-  using X = XM<ET, MB, CT>;
+  using X = XM<ET, MB, CT, NWV>;
   constexpr int NQ = X::CT / X::EPL;  // 16-byte columns per row
   if constexpr (X::F32) {
     // task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads in flight,
     // then per channel, plane and 4 inputs one 16-byte store
     const int nmq = (M + 7) / 8;
     const int ntask = NQ * nmq * X::TB;
-    for (int task = tid; task < ntask; task += 256) {
+    for (int task = tid; task < ntask; task += X::NTHR) {
       const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
       uint4 w[8];
 #pragma unroll
@@ -137,7 +139,7 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
     // task = (column q, input m, FFTs 8*o..8*o+7): 8 loads, then per
     // channel and plane one store of the 8 FFTs' components
     const int ntask = NQ * M * (X::TB / 8);
-    for (int task = tid; task < ntask; task += 256) {
+    for (int task = tid; task < ntask; task += X::NTHR) {
       const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
       uint32_t w[8][4];
 #pragma unroll
@@ -191,11 +193,12 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
 }
 
 // Channels group*CT .. group*CT+CT-1, FFT split `split`, on a workgroup of
-// 256 threads (every thread calls it with the same arguments: it contains
-// __syncthreads).  lds holds XM::LDSB bytes.
-template <int ET, int MB, int CT>
+// NWV waves, wave w taking channels w, w + NWV, ... (every thread calls it
+// with the same arguments: it contains __syncthreads).  lds holds XM::LDSB
+// bytes.
+template <int ET, int MB, int CT, int NWV = 4>
 __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int split, unsigned char *lds) {
-  using X = XM<ET, MB, CT>;
+  using X = XM<ET, MB, CT, NWV>;
   using Op = typename Operand<ET>::type;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c0 = group * X::CT;
@@ -203,7 +206,7 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   const int te = (int)((long long)p.nt * (split + 1) / p.splits);
   const int M = p.M;
 
-  for (int i = tid * 16; i < X::LDSB; i += 256 * 16) *reinterpret_cast<uint4 *>(lds + i) = make_uint4(0, 0, 0, 0);
+  for (int i = tid * 16; i < X::LDSB; i += X::NTHR * 16) *reinterpret_cast<uint4 *>(lds + i) = make_uint4(0, 0, 0, 0);
 
   floatx4 acc_re[X::CPW][X::NTILE], acc_ir[X::CPW][X::NTILE], acc_ri[X::CPW][X::NTILE];
 #pragma unroll
@@ -215,13 +218,13 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   const size_t rowbytes = (size_t)p.nchan * X::CB;
   for (int ts = tb; ts < te; ts += X::TB) {
     __syncthreads();  // the previous stage's operand reads are done
-    stage_tile<ET, MB, CT>(lds, src, rowbytes, M, c0, ts, te, tid);
+    stage_tile<ET, MB, CT, NWV>(lds, src, rowbytes, M, c0, ts, te, tid);
     __syncthreads();
-    // Wave w: channels w, w+4, ...  Lane l's operand: input l%16 of a block
+XX
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
 #pragma unroll
     for (int w = 0; w < X::CPW; w++) {
-      const unsigned char *ch = lds + (wave + 4 * w) * X::CHB;
+      const unsigned char *ch = lds + (wave + NWV * w) * X::CHB;
 #pragma unroll
       for (int ks = 0; ks < X::TB / X::K; ks++) {
         Op R[MB], I[MB];
@@ -253,7 +256,7 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   const size_t MM = (size_t)M * M;
 #pragma unroll
   for (int w = 0; w < X::CPW; w++) {
-    const int c = c0 + wave + 4 * w;
+    const int c = c0 + wave + NWV * w;
     float *pre = p.partial + ((size_t)split * p.nchan + c) * 2 * MM;
     float *pir = pre + MM;
     int tile = 0;
@@ -297,12 +300,12 @@ __device__ __forceinline__ long long xcd_swizzle(long long u, long long total) {
   return (u % kXcds) * per + u / kXcds;
 }
 
-template <int ET, int MB, int CT>
-__global__ void __launch_bounds__(256) xmfma_kernel(XMParams p) {
+template <int ET, int MB, int CT, int NWV = 4>
+__global__ void __launch_bounds__(64 * NWV) xmfma_kernel(XMParams p) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const int groups = p.nchan / CT;
   const long long u = xcd_swizzle(blockIdx.x, (long long)groups * p.splits);
-  xmfma_item<ET, MB, CT>(p, (int)(u % groups), (int)(u / groups), lds);
+  xmfma_item<ET, MB, CT, NWV>(p, (int)(u % groups), (int)(u / groups), lds);
 }
 
 // Segment of a multi-subband launch holding unit u (segments in order).
@@ -454,21 +457,21 @@ __global__ void quantise_kernel(void *out, const float2 *in, size_t n, int mode)
   }
 }
 
-template <int ET, int MB, int CT>
+template <int ET, int MB, int CT, int NWV = 4>
 hipError_t launch_xm(const XMParams &p, hipStream_t s) {
-  using X = XM<ET, MB, CT>;
+  using X = XM<ET, MB, CT, NWV>;
   if (p.nchan % X::CT) return hipErrorInvalidValue;
   const long long units = (long long)(p.nchan / X::CT) * p.splits;
   if (units > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 grid((unsigned)units);
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be allowed once per kernel
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB, CT>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB, CT, NWV>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((xmfma_kernel<ET, MB, CT>), grid, dim3(256), X::LDSB, s, p);
+  hipLaunchKernelGGL((xmfma_kernel<ET, MB, CT, NWV>), grid, dim3(X::NTHR), X::LDSB, s, p);
   return hipGetLastError();
 }
 
@@ -519,6 +522,12 @@ hipError_t launch_xm_multi_mb(const XSeg *segs, int nseg, long long total, int M
 template <int ET, int MB>
 hipError_t launch_xm_ct(const XMParams &p, int ct, hipStream_t s) {
   constexpr int D = xm_default_ct(ET, MB);
+  // 16/8-bit types at 17..32 inputs: the default tile on 8 waves of two
+  // channels each holds 111 registers instead of 240, so 4 waves share a
+  // SIMD instead of 2 (16 stations x 2 pols, FP16: 4.06 -> 3.91 ms per
+  // second of data).
+  if constexpr (ET != OUT_F32 && MB == 2)
+    if (ct == 0 || ct == D) return launch_xm<ET, MB, D, 8>(p, s);
   if (ct == 0 || ct == D) return launch_xm<ET, MB, D>(p, s);
   if (ct == D / 2) return launch_xm_if<ET, MB, D / 2>(p, s);
   if (ct == 2 * D) return launch_xm_if<ET, MB, 2 * D>(p, s);
