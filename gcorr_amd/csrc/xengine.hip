@@ -220,7 +220,7 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
     __syncthreads();  // the previous stage's operand reads are done
     stage_tile<ET, MB, CT, NWV>(lds, src, rowbytes, M, c0, ts, te, tid);
     __syncthreads();
-XX
+    // Wave w: channels w, w+NWV, ...  Lane l's operand: input l%16 of a block
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
 #pragma unroll
     for (int w = 0; w < X::CPW; w++) {
