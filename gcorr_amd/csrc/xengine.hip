@@ -457,7 +457,21 @@ __global__ void quantise_kernel(void *out, const float2 *in, size_t n, int mode)
   }
 }
 
-template <int ET, int MB, int CT, int NWV = 4>
+// Waves of a single-subband workgroup: 4, or 8 or 16 where 4 would hold
+// more than 96 accumulator registers per wave (two channels of three 16x16
+// tiles at 17..32 inputs: 111 registers, 4 waves per SIMD, where 4 waves
+// of 4 channels need 240 and leave 2).  At 16 stations x 2 pols, FP16, the
+// X-stage went from 4.06 ms (4 waves, 16 channels) to 3.91 ms (8 waves) and
+// 3.45 ms (16 waves, 32 channels: whole 128-byte rows per load).
+template <int ET, int MB, int CT>
+constexpr int xm_waves() {
+  constexpr int ntile = MB * (MB + 1) / 2;
+  for (int w : {4, 8, 16})
+    if (CT % w == 0 && (CT / w) * ntile * 12 <= 96) return w;
+  return 4;
+}
+
+template <int ET, int MB, int CT, int NWV = xm_waves<ET, MB, CT>()>
 hipError_t launch_xm(const XMParams &p, hipStream_t s) {
   using X = XM<ET, MB, CT, NWV>;
   if (p.nchan % X::CT) return hipErrorInvalidValue;
@@ -482,7 +496,8 @@ hipError_t launch_xm(const XMParams &p, hipStream_t s) {
 template <int ET, int MB, int CT>
 constexpr bool xm_tile_ok() {
   using X = XM<ET, MB, CT>;
-  return CT >= 4 && CT % X::EPL == 0 && (CT / 4) * X::NTILE * 12 <= 192 && X::LDSB <= 160 * 1024;
+  return CT >= 4 && CT % X::EPL == 0 && (CT / xm_waves<ET, MB, CT>()) * X::NTILE * 12 <= 192 &&
+         X::LDSB <= 160 * 1024;
 }
 
 template <int ET, int MB, int CT>
@@ -522,12 +537,6 @@ hipError_t launch_xm_multi_mb(const XSeg *segs, int nseg, long long total, int M
 template <int ET, int MB>
 hipError_t launch_xm_ct(const XMParams &p, int ct, hipStream_t s) {
   constexpr int D = xm_default_ct(ET, MB);
-  // 16/8-bit types at 17..32 inputs: the default tile on 8 waves of two
-  // channels each holds 111 registers instead of 240, so 4 waves share a
-  // SIMD instead of 2 (16 stations x 2 pols, FP16: 4.06 -> 3.91 ms per
-  // second of data).
-  if constexpr (ET != OUT_F32 && MB == 2)
-    if (ct == 0 || ct == D) return launch_xm<ET, MB, D, 8>(p, s);
   if (ct == 0 || ct == D) return launch_xm<ET, MB, D>(p, s);
   if (ct == D / 2) return launch_xm_if<ET, MB, D / 2>(p, s);
   if (ct == 2 * D) return launch_xm_if<ET, MB, 2 * D>(p, s);
