@@ -457,18 +457,23 @@ __global__ void quantise_kernel(void *out, const float2 *in, size_t n, int mode)
   }
 }
 
-// Waves of a single-subband workgroup: 4, or 8 or 16 where 4 would hold
-// more than 96 accumulator registers per wave (two channels of three 16x16
-// tiles at 17..32 inputs: 111 registers, 4 waves per SIMD, where 4 waves
-// of 4 channels need 240 and leave 2).  At 16 stations x 2 pols, FP16, the
+// Waves of a single-subband workgroup: 4, or 8 or 16 (as many as the tile
+// has channels for) where 4 would hold more than 96 accumulator registers
+// per wave (two channels of three 16x16 tiles at 17..32 inputs: 111
+// registers, 4 waves per SIMD, where 4 waves of 4 channels need 240 and
+// leave 2).  At 16 stations x 2 pols, FP16, the
 // X-stage went from 4.06 ms (4 waves, 16 channels) to 3.91 ms (8 waves) and
 // 3.45 ms (16 waves, 32 channels: whole 128-byte rows per load).
 template <int ET, int MB, int CT>
 constexpr int xm_waves() {
   constexpr int ntile = MB * (MB + 1) / 2;
-  for (int w : {4, 8, 16})
-    if (CT % w == 0 && (CT / w) * ntile * 12 <= 96) return w;
-  return 4;
+  int best = 4;
+  for (int w : {4, 8, 16}) {
+    if (CT % w) break;
+    best = w;
+    if ((CT / w) * ntile * 12 <= 96) break;
+  }
+  return best;
 }
 
 template <int ET, int MB, int CT, int NWV = xm_waves<ET, MB, CT>()>
