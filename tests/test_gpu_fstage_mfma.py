@@ -112,6 +112,22 @@ def test_mfma_fstage_fp32_invalid_ffts(gpu):
     assert eps(v, ref) < 1e-5
 
 
+@pytest.mark.parametrize("precision,bound", [("fp16", 1e-3), ("fp32", 1e-5)])
+def test_mfma_fstage_few_units_and_chunks(gpu, precision, bound):
+    """Two stations and 5 FFTs (fewer units than workgroups of the
+    persistent grid), then the same subintegration in 3 uneven chunks
+    (chunk_ffts = 2): every unit of every chunk is still correlated once."""
+    w, data, poly, offs = case(2, 2048, 5, seed=66)
+    ref, rc = fxref.process(fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True),
+                            data, poly, offs, "f64")
+    v, c = run(w, data, poly, offs, precision=precision, fblock=MFMA)
+    np.testing.assert_array_equal(c, rc)
+    assert eps(v, ref) < bound
+    v2, c2 = run(w, data, poly, offs, precision=precision, fblock=MFMA, chunk_ffts=2)
+    np.testing.assert_array_equal(c2, rc)
+    assert eps(v2, ref) < bound
+
+
 def test_mfma_fstage_rejected_where_unsupported(gpu):
     with pytest.raises(gcorr_amd.GxcError):  # FP32 form: up to 4096 points
         gcorr_amd.FxKernel(3, 4096, 10, 2, 1.65e9, 64e6, precision="fp32", fblock=MFMA)
