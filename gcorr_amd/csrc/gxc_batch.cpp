@@ -186,9 +186,10 @@ gxc_status build_tables(gxc_batch *b) {
   b->d_scaleseg = nullptr;
   const int ns = (int)b->sub.size(), C = b->nchunks;
   const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
-  auto on_mfma = [&](const Sub &u) {
-    return b->cfg.fstage_mfma && gxc::fstage_mfma_supported(u.d.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32);
-  };
+  // with fstage_mfma, the subbands of 2048 points and more get launches of
+  // their own: on the matrix cores where those cover them, else on the
+  // one-pol kernel
+  auto on_mfma = [&](const Sub &u) { return b->cfg.fstage_mfma && u.d.L >= 2048; };
   b->nmulti = 0;
   for (const Sub &u : b->sub) b->nmulti += on_mfma(u) ? 0 : 1;
   b->mseg.assign(C, {});
@@ -332,7 +333,10 @@ gxc_status enqueue_device_work(gxc_batch *b) {
     if (b->nmulti > 0 && b->ftotal[c] > 0)
       GXB_HIP(b, gxc::launch_fstage_multi(b->d_fseg + (size_t)c * b->nmulti, b->nmulti, b->ftotal[c], b->out_mode,
                                           b->fblock, fs));
-    for (const gxc::FSeg &f : b->mseg[c]) GXB_HIP(b, gxc::launch_fstage_mfma(f.p, f.L, fs));
+    for (const gxc::FSeg &f : b->mseg[c])
+      GXB_HIP(b, gxc::fstage_mfma_supported(f.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32)
+                     ? gxc::launch_fstage_mfma(f.p, f.L, fs)
+                     : gxc::launch_fstage_big(f.p, f.L, b->cfg.is_complex != 0, fs));
     mark(b, 1, fs);
     if (overlap) {
       GXB_HIP(b, hipEventRecord(b->ev_f[buf], fs));
@@ -745,10 +749,8 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
   gxc_status st = time_process(b, &def_ms);
   cand++;
   int best_fb = b->fblock, best_units = b->xunits, best_mfma = b->cfg.fstage_mfma;
-  bool any_mfma = false;
-  for (const Sub &u : b->sub)
-    any_mfma = any_mfma ||
-               gxc::fstage_mfma_supported(u.d.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32);
+  bool any_mfma = false;  // a subband that fstage_mfma would give its own launch
+  for (const Sub &u : b->sub) any_mfma = any_mfma || u.d.L >= 2048;
   for (int mf = 0; mf <= (any_mfma ? 1 : 0); mf++)
     for (int fb : {256, 512}) {
       if (st != GXC_OK) continue;

@@ -402,10 +402,12 @@ typedef struct {
                                    0 = automatic */
   int overlap;                  /* as gxc_config.overlap */
   int graph;                    /* as gxc_config.graph */
-  int fstage_mfma;              /* 1: subbands the matrix-core F-stage covers
-                                   (GXC_FBLOCK_MFMA) run on it, one launch
-                                   each; the others stay in the multi-subband
-                                   launch.  gxc_batch_autotune chooses. */
+  int fstage_mfma;              /* 1: subbands of 2048 points and more get a
+                                   launch each, on the matrix-core F-stage
+                                   (GXC_FBLOCK_MFMA) where it covers them,
+                                   else on the one-pol F-stage; the others
+                                   stay in the multi-subband launch.
+                                   gxc_batch_autotune chooses. */
   int reserved[3];
 } gxc_batch_config;
 
@@ -433,11 +435,12 @@ gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n
 /* Autotuner of a batch (input and delays of every subband set): times whole
  * subintegrations for each F-stage workgroup size (256/512 threads) and X
  * unit count (1024..8192, which sets every subband's FFT splits), with and
- * without the matrix-core F-stage where it covers a subband, then with and
- * without graph replay, and keeps the fastest.  In the result, fblock is the
- * multi-subband workgroup size, or GXC_FBLOCK_MFMA when the matrix-core
- * F-stage was chosen (the choice is kept in the batch); xsplits holds the
- * chosen X unit count and fstage_ms/xstage_ms are 0. */
+ * without launches of their own for the subbands of 2048 points and more
+ * (gxc_batch_config.fstage_mfma), then with and without graph replay, and
+ * keeps the fastest.  In the result, fblock is the multi-subband workgroup
+ * size, or GXC_FBLOCK_MFMA when the own launches were chosen (the choice is
+ * kept in the batch); xsplits holds the chosen X unit count and
+ * fstage_ms/xstage_ms are 0. */
 gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out);
 
 /* Timing over all subbands (launch counts are per-stage launches, one per

@@ -81,8 +81,9 @@ def test_batch_fp16_vs_fp64_oracle(gpu):
 
 @pytest.mark.parametrize("precision,bound,mode", [("fp32", EPS, "f32"), ("fp16", 1e-3, "f64")])
 def test_batch_matrix_core_fstage(gpu, precision, bound, mode):
-    """fstage_mfma: the 2048..8192-point subbands (2048..4096 for FP32) on the
-    matrix-core F-stage, one launch each, the others in the multi launch."""
+    """fstage_mfma: the 2048..8192-point subbands get a launch each, on the
+    matrix-core F-stage (2048..4096 points for FP32) or else the one-pol
+    F-stage; the others stay in the multi launch."""
     ws = subbands(div=125)
     gen = [synth.generate(w, device="cpu") for w in ws]
     datas = [[b.numpy() for b in g[0]] for g in gen]
