@@ -179,6 +179,7 @@ def load():
         "gxc_batch_set_input": (ctypes.c_int, [H, ctypes.c_int, P(u8p)]),
         "gxc_batch_set_input_device": (ctypes.c_int, [H, ctypes.c_int, P(vp)]),
         "gxc_batch_set_delays": (ctypes.c_int, [H, ctypes.c_int, P(dp), dp]),
+        "gxc_batch_set_pol_conversion": (ctypes.c_int, [H, ctypes.c_int, vp]),
         "gxc_batch_process": (ctypes.c_int, [H]),
         "gxc_batch_process_async": (ctypes.c_int, [H]),
         "gxc_batch_synchronize": (ctypes.c_int, [H]),

@@ -93,6 +93,17 @@ class Batch:
                                                  f.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                     self._h)
 
+    def set_pol_conversion(self, sub, gains):
+        """gains: complex [nant][2][nchan of subband sub], or None to turn the
+        conversion off (gxc_batch_set_pol_conversion)."""
+        if gains is None:
+            check_batch(self._L.gxc_batch_set_pol_conversion(self._h, int(sub), None), self._h)
+            return
+        n = self.subbands[sub][0]
+        g = np.ascontiguousarray(gains, np.complex64).reshape(self.nant, 2, n)
+        check_batch(self._L.gxc_batch_set_pol_conversion(self._h, int(sub), g.ctypes.data_as(ctypes.c_void_p)),
+                    self._h)
+
     def process(self):
         check_batch(self._L.gxc_batch_process(self._h), self._h)
 

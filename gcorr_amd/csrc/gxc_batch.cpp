@@ -51,6 +51,9 @@ struct Sub {
   int *d_counts = nullptr;
   float2 *d_vis = nullptr;
   float *d_partial = nullptr;
+  // polarisation conversion of the visibilities (gxc_batch_set_pol_conversion)
+  bool polconv = false;
+  float2 *d_polgains = nullptr;  // [nant][2][nchan]
   float *d_scales = nullptr, *d_power = nullptr;
   size_t spec_off = 0;  // byte offset of this subband in a spectra buffer
 };
@@ -350,6 +353,9 @@ gxc_status enqueue_device_work(gxc_batch *b) {
   mark(b, 4, xs);
   GXB_HIP(b, gxc::launch_counts_multi(b->d_countseg, ns, b->counttotal, xs));
   GXB_HIP(b, gxc::launch_finalise_multi(b->d_finseg, ns, b->fintotal, b->M, xs));
+  for (const Sub &u : b->sub)
+    if (u.polconv)
+      GXB_HIP(b, gxc::launch_polconv(u.d_vis, u.d_polgains, b->d_rowmap, b->nrows, u.cfg.nchan, xs));
   mark(b, 5, xs);
   if (overlap) {
     GXB_HIP(b, hipEventRecord(b->ev_join, xs));
@@ -595,6 +601,7 @@ void gxc_batch_destroy(gxc_batch *b) {
     hipFree(u.d_counts);
     hipFree(u.d_vis);
     hipFree(u.d_partial);
+    hipFree(u.d_polgains);
     hipFree(u.d_scales);
     hipFree(u.d_power);
   }
@@ -651,6 +658,24 @@ gxc_status gxc_batch_set_delays(gxc_batch *b, int sub, const double *const *poly
   GXB_HIP(b, hipMemcpy(u->d_delays, u->h_delays.data(), u->h_delays.size() * sizeof(double),
                        hipMemcpyHostToDevice));
   u->delays_set = true;
+  return GXC_OK;
+}
+
+gxc_status gxc_batch_set_pol_conversion(gxc_batch *b, int sub, const gxc_cf32 *gains) {
+  Sub *u = get_sub(b, sub);
+  if (!u) return set_err(b, GXC_ERR_INVALID_ARG, "bad subband");
+  GXB_HIP(b, hipStreamSynchronize(b->stream));
+  const bool on = gains != nullptr;
+  if (on) {
+    const size_t n = (size_t)b->cfg.nant * 2 * u->cfg.nchan;
+    if (!u->d_polgains) GXB_TRY(dmalloc(b, &u->d_polgains, n));
+    GXB_HIP(b, hipMemcpy(u->d_polgains, gains, n * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  if (on != u->polconv && b->graph_exec) {  // the captured work changes
+    hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  u->polconv = on;
   return GXC_OK;
 }
 

@@ -425,6 +425,10 @@ gxc_status gxc_batch_set_input(gxc_batch *b, int sub, const uint8_t *const *per_
 gxc_status gxc_batch_set_input_device(gxc_batch *b, int sub, const void *const *per_station);
 gxc_status gxc_batch_set_delays(gxc_batch *b, int sub, const double *const *poly3,
                                 const double *fileoffsets);
+/* Polarisation conversion of subband `sub`, as gxc_set_pol_conversion
+ * (examples/chris/bench/gpubench.cu:141-166): gains [nant][2][nchan of that
+ * subband], copied at this call; NULL turns it off. */
+gxc_status gxc_batch_set_pol_conversion(gxc_batch *b, int sub, const gxc_cf32 *gains);
 /* One subintegration of every subband. */
 gxc_status gxc_batch_process(gxc_batch *b);
 gxc_status gxc_batch_process_async(gxc_batch *b);

@@ -114,6 +114,43 @@ def test_batch_matrix_core_modes_agree(gpu):
                 np.testing.assert_array_equal(v, v0)
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_pol_conversion(gpu, graph):
+    """gxc_batch_set_pol_conversion on some subbands: those match the
+    oracle's per-spectrum conversion (gpubench.cu:141-166) within the FP32
+    bound, the others stay unconverted; turning it off restores them."""
+    ws = subbands(div=500)
+    datas = random_inputs(ws, seed=8)
+    delays = [synth.make_delays(w) for w in ws]
+    rng = np.random.default_rng(8)
+    gains = {s: ((rng.standard_normal((w.nant, 2, w.nchan)) + 1j * rng.standard_normal((w.nant, 2, w.nchan)))
+                 / np.sqrt(2)).astype(np.complex64) for s, w in enumerate(ws) if s % 3 == 1}
+    b = gcorr_amd.Batch(8, [(w.nchan, w.numffts, w.lo, w.bw) for w in ws], autos=True, graph=graph)
+    for s in range(len(ws)):
+        b.set_input(s, datas[s])
+        b.set_delays(s, *delays[s])
+    b.process()
+    plain = [b.visibilities(s) for s in range(len(ws))]
+    for s, g in gains.items():
+        b.set_pol_conversion(s, g)
+    b.process()
+    for s, w in enumerate(ws):
+        v = b.visibilities(s)
+        if s in gains:
+            cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=2, autos=True, polgains=gains[s])
+            ref, rc = fxref.process(cfg, datas[s], *delays[s])
+            np.testing.assert_array_equal(b.row_counts(s), rc)
+            assert eps(v, ref) < EPS, s
+        else:
+            np.testing.assert_array_equal(v, plain[s])
+    for s in gains:
+        b.set_pol_conversion(s, None)
+    b.process()
+    for s in range(len(ws)):
+        np.testing.assert_array_equal(b.visibilities(s), plain[s])
+    b.close()
+
+
 def test_batch_modes_agree(gpu):
     """Overlap and graph replay change scheduling only: identical results."""
     ws = subbands(div=500)
