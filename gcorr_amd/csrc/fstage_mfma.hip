@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   dft_operand<RL, O>(c16, g, RL == 32 ? 0 : c16 >> 3, l_rh, l_rl, l_ih, l_il);
 
   // Twiddle tables, the same for every unit (f16 pairs, widened exactly in
-  // twmul).  Pass-0 output row r of the column tile holding column b takes
+  // twmul; FP32 for the FP32 form).  Pass-0 output row r of the column tile holding column b takes
   // W_L^(r RL (b / RL)) = W_256^(r s), s = b / RL < 16: st0[16 s + r].
   // Pass-1 output q1 = 256 t + c + 16 r (tile t, column c, row r = 4 g + i)
   // takes W_L^((c + 16 r) t): st1[256 t + 16 c + r].
