@@ -111,6 +111,38 @@ def test_vdif_ingest_matches_oracle(gpu, graph):
     k.close()
 
 
+@pytest.mark.parametrize("precision,bound,mode", [("fp16", 1e-3, "f64"), ("fp32", EPS, "f32")])
+def test_vdif_missing_frames_matrix_core_fstage(gpu, precision, bound, mode):
+    """Missing frames invalidate FFTs on the matrix-core F-stage too (its
+    delay splits, frame checks included, are computed a batch of units at a
+    time)."""
+    w, data, poly, offs, rng = case(3, 1024, 60, seed=33)
+    payload, fps, s0, f0 = 4000, 100, 55, 2
+    drops = [[1], [], [6, 7]]
+    streams = [damaged_stream(data[a], payload, rng, 2, False, fps, s0, f0, drops[a])
+               for a in range(w.nant)]
+    cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True)
+    placed, flags, spfs = [], [], []
+    for st in streams:
+        buf, ok, spf, _ = fxref.vdif_place(cfg, st, frames_per_second=fps, ref_second=s0, ref_frame=f0)
+        placed.append(buf)
+        flags.append(ok)
+        spfs.append(spf)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True, precision=precision,
+                           fblock=-1)
+    ingest_all(k, streams, fps, s0, f0)
+    k.setDelays(poly, offs)
+    k.process()
+    v, c = k.visibilities(), k.row_counts()
+    cfgm = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True, frame_ok=flags,
+                             frame_samples=spfs)
+    ref, rc = fxref.process(cfgm, placed, poly, offs, mode)
+    np.testing.assert_array_equal(c, rc)
+    assert c.min() < w.numffts
+    assert eps(v, ref) < bound
+    k.close()
+
+
 def test_vdif_complete_stream_equals_plain_input(gpu):
     w, data, poly, offs, rng = case(3, 512, 80, seed=32)
     fps = 250
