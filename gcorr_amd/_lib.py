@@ -180,6 +180,10 @@ def load():
         "gxc_batch_set_input_device": (ctypes.c_int, [H, ctypes.c_int, P(vp)]),
         "gxc_batch_set_delays": (ctypes.c_int, [H, ctypes.c_int, P(dp), dp]),
         "gxc_batch_set_pol_conversion": (ctypes.c_int, [H, ctypes.c_int, vp]),
+        "gxc_batch_set_input_vdif": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t,
+                                                    P(VdifSpec), P(VdifStats)]),
+        "gxc_batch_get_frame_flags": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, u8p, ctypes.c_size_t,
+                                                     P(ctypes.c_longlong)]),
         "gxc_batch_process": (ctypes.c_int, [H]),
         "gxc_batch_process_async": (ctypes.c_int, [H]),
         "gxc_batch_synchronize": (ctypes.c_int, [H]),

@@ -93,6 +93,36 @@ class Batch:
                                                  f.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                     self._h)
 
+    def set_input_vdif(self, sub, station, dev_ptr, nbytes, *, frames_per_second, ref_second=0,
+                       ref_frame=0, ref_epoch=0, thread_id=0, frame_bytes=0, offset_binary8=True,
+                       keepalive=None, stats=True):
+        """Station `station` of subband `sub` from VDIF frames in device memory
+        (gxc_batch_set_input_vdif); as FxKernel.setInputVDIF."""
+        spec = _lib.VdifSpec()
+        self._L.gxc_vdif_spec_init(ctypes.byref(spec))
+        spec.frame_bytes, spec.thread_id = int(frame_bytes), int(thread_id)
+        spec.frames_per_second, spec.ref_epoch = int(frames_per_second), int(ref_epoch)
+        spec.ref_second, spec.ref_frame = int(ref_second), int(ref_frame)
+        spec.offset_binary8 = int(bool(offset_binary8))
+        st = _lib.VdifStats()
+        check_batch(self._L.gxc_batch_set_input_vdif(self._h, int(sub), int(station), ctypes.c_void_p(int(dev_ptr)),
+                                                     int(nbytes), ctypes.byref(spec),
+                                                     ctypes.byref(st) if stats else None), self._h)
+        # the ingest is queued: the frames must live until it has run
+        self._keep[("vdif", int(sub), int(station))] = keepalive
+        return {k: getattr(st, k) for k, _ in _lib.VdifStats._fields_} if stats else None
+
+    def frame_flags(self, sub, station):
+        """Per-slot presence flags of the last VDIF ingest of (sub, station)."""
+        n = ctypes.c_longlong(0)
+        check_batch(self._L.gxc_batch_get_frame_flags(self._h, int(sub), int(station), None, 0, ctypes.byref(n)),
+                    self._h)
+        out = np.empty(n.value, np.uint8)
+        check_batch(self._L.gxc_batch_get_frame_flags(self._h, int(sub), int(station),
+                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), out.size,
+                                                      None), self._h)
+        return out
+
     def set_pol_conversion(self, sub, gains):
         """gains: complex [nant][2][nchan of subband sub], or None to turn the
         conversion off (gxc_batch_set_pol_conversion)."""

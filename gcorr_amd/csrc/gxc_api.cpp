@@ -81,8 +81,8 @@ gxc_status run_fstage(gxc_handle *h, int t0, int nt, int mode, void *out, int ts
   p.scales = h->d_scales;
   p.power = h->d_power;
   p.valid = h->d_valid;
-  p.frame_ok = h->vdif ? h->d_flag_ptrs : nullptr;
-  p.frame_samples = h->d_spf;
+  p.frame_ok = h->vdif ? h->vs.d_flag_ptrs : nullptr;
+  p.frame_samples = h->vs.d_spf;
   p.out_mode = mode;
   p.t0 = t0;
   p.nt = nt;
@@ -424,13 +424,8 @@ void gxc_destroy(gxc_handle *h) {
   hipEvent_t evs[6] = {h->ev_fork, h->ev_join, h->ev_f[0], h->ev_f[1], h->ev_x[0], h->ev_x[1]};
   for (hipEvent_t e : evs)
     if (e) hipEventDestroy(e);
-  for (uint8_t *f : h->d_flags) hipFree(f);
-  hipFree(h->d_flag_ptrs);
+  h->vs.release();
   hipFree(h->d_polgains);
-  hipFree(h->d_spf);
-  hipFree(h->d_nslots);
-  hipFree(h->d_slot_src);
-  hipFree(h->d_vstats);
   if (h->cstream) hipStreamSynchronize(h->cstream);
   for (auto &sl : h->slot) {
     hipFree(sl.d_in);

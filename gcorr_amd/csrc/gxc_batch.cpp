@@ -22,6 +22,7 @@
 #include "../../include/gxcorr.h"
 #include "gxc_host.hpp"
 #include "gxc_internal.hpp"
+#include "gxc_vdif_state.hpp"
 #include "gxc_xengine.hpp"
 
 namespace {
@@ -54,6 +55,10 @@ struct Sub {
   // polarisation conversion of the visibilities (gxc_batch_set_pol_conversion)
   bool polconv = false;
   float2 *d_polgains = nullptr;  // [nant][2][nchan]
+  // VDIF ingest (gxc_batch_set_input_vdif): payloads go into d_input_owned,
+  // and the frame flags make FFTs over missing frames invalid
+  bool vdif = false;
+  gxc::host::VdifState vs;
   float *d_scales = nullptr, *d_power = nullptr;
   size_t spec_off = 0;  // byte offset of this subband in a spectra buffer
 };
@@ -168,6 +173,8 @@ gxc::FParams fparams(const gxc_batch *b, const Sub &u, int t0, int nt, int tstri
   p.lofrac = lo - std::trunc(lo);
   p.fraclo = (lo - std::trunc(lo)) > 0.00000000001 ? 1 : 0;  // TINY, fxkernel.cpp:66-71
   p.sampletime = u.d.sampletime;
+  p.frame_ok = u.vdif ? u.vs.d_flag_ptrs : nullptr;
+  p.frame_samples = u.vs.d_spf;
   return p;
 }
 
@@ -373,6 +380,18 @@ gxc_status ensure_owned_input(gxc_batch *b, Sub &u) {
   return GXC_OK;
 }
 
+// The segment tables hold each subband's frame-flag pointers: rebuild them,
+// and drop a captured graph, when a subband enters or leaves VDIF mode.
+gxc_status vdif_mode(gxc_batch *b, Sub &u, bool on) {
+  if (u.vdif == on) return GXC_OK;
+  u.vdif = on;
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  b->graph_exec = nullptr;
+  return build_tables(b);
+}
+
+gxc_status leave_vdif(gxc_batch *b, Sub &u) { return vdif_mode(b, u, false); }
+
 gxc_status point_at_owned(gxc_batch *b, Sub &u) {
   std::vector<const uint8_t *> ptrs(b->cfg.nant);
   for (int a = 0; a < b->cfg.nant; a++) ptrs[a] = u.d_input_owned + (size_t)a * u.d.station_bytes;
@@ -389,7 +408,7 @@ gxc_status enqueue_process(gxc_batch *b) {
   b->marks.clear();
   mark(b, 7, b->stream);
   for (Sub &u : b->sub) {
-    if (u.input_device) continue;
+    if (u.input_device || u.vdif) continue;
     for (int a = 0; a < b->cfg.nant; a++)
       GXB_HIP(b, hipMemcpyAsync(u.d_input_owned + (size_t)a * u.d.station_bytes, u.host_in[a], u.d.station_bytes,
                                 hipMemcpyHostToDevice, b->stream));
@@ -602,6 +621,7 @@ void gxc_batch_destroy(gxc_batch *b) {
     hipFree(u.d_vis);
     hipFree(u.d_partial);
     hipFree(u.d_polgains);
+    u.vs.release();
     hipFree(u.d_scales);
     hipFree(u.d_power);
   }
@@ -625,6 +645,7 @@ gxc_status gxc_batch_set_input(gxc_batch *b, int sub, const uint8_t *const *per_
   for (const uint8_t *p : in)
     if (!p) return set_err(b, GXC_ERR_INVALID_ARG, "null station buffer");
   GXB_HIP(b, hipStreamSynchronize(b->stream));
+  GXB_TRY(leave_vdif(b, *u));
   GXB_TRY(ensure_owned_input(b, *u));
   if (u->input_device || !u->input_set) GXB_TRY(point_at_owned(b, *u));
   u->host_in = in;
@@ -640,6 +661,7 @@ gxc_status gxc_batch_set_input_device(gxc_batch *b, int sub, const void *const *
   for (const void *p : in)
     if (!p) return set_err(b, GXC_ERR_INVALID_ARG, "null station buffer");
   GXB_HIP(b, hipStreamSynchronize(b->stream));
+  GXB_TRY(leave_vdif(b, *u));
   GXB_HIP(b, hipMemcpy(u->d_station_ptrs, in.data(), in.size() * sizeof(void *), hipMemcpyHostToDevice));
   u->input_device = true;
   u->input_set = true;
@@ -659,6 +681,48 @@ gxc_status gxc_batch_set_delays(gxc_batch *b, int sub, const double *const *poly
                        hipMemcpyHostToDevice));
   u->delays_set = true;
   return GXC_OK;
+}
+
+gxc_status gxc_batch_set_input_vdif(gxc_batch *b, int sub, int station, const void *d_frames, size_t nbytes,
+                                    const gxc_vdif_spec *spec, gxc_vdif_stats *stats) {
+  Sub *u = get_sub(b, sub);
+  if (!u) return set_err(b, GXC_ERR_INVALID_ARG, "bad subband");
+  const int nant = b->cfg.nant;
+  if (station < 0 || station >= nant) return set_err(b, GXC_ERR_INVALID_ARG, "station out of range");
+  gxc::host::VdifCall call;
+  std::string why;
+  gxc_status st = gxc::host::vdif_prepare(&u->cfg, u->d.station_bytes, d_frames, nbytes, spec, b->stream, &call, &why);
+  if (st != GXC_OK) return set_err(b, st, why);
+  if (gxc::host::vdif_needs_growth(u->vs, station, call) || !u->vdif || !u->d_input_owned) {
+    // buffers change under work that may still be queued: wait for it
+    GXB_HIP(b, hipStreamSynchronize(b->stream));
+  }
+  GXB_TRY(ensure_owned_input(b, *u));
+  st = gxc::host::vdif_reserve(u->vs, nant, station, call, &why);
+  if (st != GXC_OK) return set_err(b, st, why);
+  if (!u->vdif) {
+    // enter VDIF mode: station pointers to the owned buffer, flags on
+    GXB_TRY(point_at_owned(b, *u));
+    u->input_device = false;
+    u->input_set = false;
+    u->vs.done.assign(nant, 0);
+    GXB_TRY(vdif_mode(b, *u, true));
+  }
+  st = gxc::host::vdif_ingest(u->vs, &u->cfg, station, u->d_input_owned + (size_t)station * u->d.station_bytes,
+                              u->d.station_bytes, d_frames, call, spec, stats, b->stream, &why);
+  if (st != GXC_OK) return set_err(b, st, why);
+  u->input_set = std::all_of(u->vs.done.begin(), u->vs.done.end(), [](char c) { return c != 0; });
+  return GXC_OK;
+}
+
+gxc_status gxc_batch_get_frame_flags(gxc_batch *b, int sub, int station, uint8_t *out, size_t n, long long *nslots) {
+  Sub *u = get_sub(b, sub);
+  if (!u) return set_err(b, GXC_ERR_INVALID_ARG, "bad subband");
+  if (station < 0 || station >= b->cfg.nant) return set_err(b, GXC_ERR_INVALID_ARG, "station out of range");
+  if (!u->vdif) return set_err(b, GXC_ERR_STATE, "station has no VDIF ingest");
+  std::string why;
+  const gxc_status st = gxc::host::vdif_flags(u->vs, station, out, n, nslots, b->stream, &why);
+  return st == GXC_OK ? GXC_OK : set_err(b, st, why);
 }
 
 gxc_status gxc_batch_set_pol_conversion(gxc_batch *b, int sub, const gxc_cf32 *gains) {

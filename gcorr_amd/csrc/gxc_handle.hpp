@@ -15,6 +15,7 @@
 
 #include "gxc_host.hpp"
 #include "gxc_internal.hpp"
+#include "gxc_vdif_state.hpp"
 #include "gxc_xengine.hpp"
 
 struct gxc_handle {
@@ -34,16 +35,8 @@ struct gxc_handle {
 
   // VDIF ingest (gxc_set_input_vdif): payloads go into d_input_owned, and
   // the per-station frame flags make FFTs over missing frames invalid
-  bool vdif = false;                   // input comes from gxc_set_input_vdif
-  std::vector<char> vdif_done;         // stations ingested since the switch
-  std::vector<uint8_t *> d_flags;      // [nant] frame flags, capacity flag_cap
-  std::vector<long long> flag_cap;
-  const uint8_t **d_flag_ptrs = nullptr;  // [nant] = d_flags
-  int *d_spf = nullptr;                   // [nant] time samples per frame (set by the ingest)
-  long long *d_nslots = nullptr;          // [nant] slots of the last ingest (set by the ingest)
-  unsigned *d_slot_src = nullptr;         // [slot_cap] ingest scratch
-  long long slot_cap = 0;
-  unsigned long long *d_vstats = nullptr;  // [kVdifCounters]
+  bool vdif = false;  // input comes from gxc_set_input_vdif
+  gxc::host::VdifState vs;
 
   // polarisation conversion of the visibilities (gxc_set_pol_conversion)
   bool polconv = false;

@@ -425,6 +425,15 @@ gxc_status gxc_batch_set_input(gxc_batch *b, int sub, const uint8_t *const *per_
 gxc_status gxc_batch_set_input_device(gxc_batch *b, int sub, const void *const *per_station);
 gxc_status gxc_batch_set_delays(gxc_batch *b, int sub, const double *const *poly3,
                                 const double *fileoffsets);
+/* Station `station` of subband `sub` from VDIF frames in device memory, as
+ * gxc_set_input_vdif (gcorr's ingest, ingest/src/ingest.c:267-355): the
+ * subband's frames are those of spec->thread_id; missing frames make the
+ * FFTs over them invalid.  gxc_batch_set_input / _device leave VDIF mode. */
+gxc_status gxc_batch_set_input_vdif(gxc_batch *b, int sub, int station, const void *d_frames, size_t nbytes,
+                                    const gxc_vdif_spec *spec, gxc_vdif_stats *stats);
+/* gxc_get_frame_flags for station `station` of subband `sub`. */
+gxc_status gxc_batch_get_frame_flags(gxc_batch *b, int sub, int station, uint8_t *out, size_t n,
+                                     long long *nslots);
 /* Polarisation conversion of subband `sub`, as gxc_set_pol_conversion
  * (examples/chris/bench/gpubench.cu:141-166): gains [nant][2][nchan of that
  * subband], copied at this call; NULL turns it off. */

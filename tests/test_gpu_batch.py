@@ -151,6 +151,65 @@ def test_batch_pol_conversion(gpu, graph):
     b.close()
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_vdif_input(gpu, graph):
+    """gxc_batch_set_input_vdif on two subbands: their frame flags equal the
+    oracle's placement and their visibilities the oracle's run on the placed
+    buffers with the missing-frame flags (FP32 bound); the other subbands are
+    unchanged; set_input leaves VDIF mode again."""
+    import torch
+    from gcorr_amd import vdif
+
+    ws = subbands(div=500)
+    datas = random_inputs(ws, seed=9)
+    delays = [synth.make_delays(w) for w in ws]
+    b = gcorr_amd.Batch(8, [(w.nchan, w.numffts, w.lo, w.bw) for w in ws], autos=True, graph=graph)
+    for s in range(len(ws)):
+        b.set_input(s, datas[s])
+        b.set_delays(s, *delays[s])
+    b.process()
+    plain = [b.visibilities(s) for s in range(len(ws))]
+    fps, s0, f0 = 1000, 7, 3
+    for s in (2, 5):
+        w = ws[s]
+        payload = 1024
+        cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=2, autos=True)
+        placed, flags, spfs = [], [], []
+        for a in range(w.nant):
+            fr = vdif.frames_from_station(datas[s][a], payload, frames_per_second=fps, ref_second=s0,
+                                          ref_frame=f0, thread=s)
+            keep = [k for k in range(fr.shape[0]) if not (a == 1 and k in (2, 3))]
+            stream = fr[keep].reshape(-1)
+            buf, ok, spf, _ = fxref.vdif_place(cfg, stream, frames_per_second=fps, ref_second=s0, ref_frame=f0,
+                                               thread_id=s)
+            placed.append(buf)
+            flags.append(ok)
+            spfs.append(spf)
+            dev = torch.from_numpy(stream).to("cuda")
+            b.set_input_vdif(s, a, dev.data_ptr(), stream.size, frames_per_second=fps, ref_second=s0,
+                             ref_frame=f0, thread_id=s, keepalive=dev)
+        torch.cuda.synchronize()
+        for a in range(w.nant):
+            np.testing.assert_array_equal(b.frame_flags(s, a), flags[a])
+        ws_ref = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, nbit=2, autos=True, frame_ok=flags,
+                                   frame_samples=spfs)
+        delays_s = delays[s]
+        b.process()
+        ref, rc = fxref.process(ws_ref, placed, *delays_s)
+        np.testing.assert_array_equal(b.row_counts(s), rc)
+        assert b.row_counts(s).min() < w.numffts
+        assert eps(b.visibilities(s), ref) < EPS, s
+    for s in range(len(ws)):
+        if s not in (2, 5):
+            np.testing.assert_array_equal(b.visibilities(s), plain[s])
+    for s in (2, 5):
+        b.set_input(s, datas[s])
+    b.process()
+    for s in range(len(ws)):
+        np.testing.assert_array_equal(b.visibilities(s), plain[s])
+    b.close()
+
+
 def test_batch_modes_agree(gpu):
     """Overlap and graph replay change scheduling only: identical results."""
     ws = subbands(div=500)
