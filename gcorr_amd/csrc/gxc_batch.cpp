@@ -551,14 +551,14 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
       return fail(set_err(b, GXC_ERR_HIP, "hipEventCreate failed"));
 
   // chunks: automatic = 8 passes with overlap, else as few as keep one
-  // spectra buffer within 8 GiB; at most the smallest subband's FFT count,
+  // spectra buffer within 32 GiB; at most the smallest subband's FFT count,
   // so every chunk has FFTs of every subband
   int C = cfg->chunks;
   if (C <= 0) {
     double total = 0;
     for (const Sub &u : b->sub)
       total += (double)u.cfg.numffts * 2 * cfg->nant * u.cfg.nchan * gxc::xm_element_bytes(out_mode);
-    C = cfg->overlap ? 8 : (int)std::ceil(total / (8.0 * (1ull << 30)));
+    C = cfg->overlap ? 8 : (int)std::ceil(total / (32.0 * (1ull << 30)));
   }
   C = std::max(1, std::min(C, minfft));
   b->nchunks = C;
