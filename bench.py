@@ -51,17 +51,18 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(workload, precision, fblock):
+def traffic_from_profiles(workload, precision, fblock, launches_per_step):
     """HBM bytes per launch of each stage measured with rocprofv3 --pmc
     (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy), committed as
     profiles/traffic_<workload>_<precision>.json for the F-stage workgroup
-    size the autotuner chose there; None when this run chose another."""
+    size the autotuner chose there and the F-stage launches per step of that
+    run; None when this run chose another shape or another chunking."""
     p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (workload, precision))
     if not os.path.exists(p):
         return None
     with open(p) as f:
         t = json.load(f)
-    if t.get("fblock") != fblock:
+    if t.get("fblock") != fblock or t.get("fstage_launches_per_step") != launches_per_step:
         return None
     return t
 
@@ -208,7 +209,8 @@ def main():
     }
     dom = "fstage" if f_ms >= x_ms else "xstage"
     achieved = stages[dom]["GBps"]
-    traffic = traffic_from_profiles(args.workload, args.precision, (tuning or {}).get("fblock"))
+    traffic = traffic_from_profiles(args.workload, args.precision, (tuning or {}).get("fblock"),
+                                    f_launch / args.steps)
     roofline = {
         "kernel": dom,
         "bound": "hbm",
