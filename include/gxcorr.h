@@ -79,7 +79,11 @@ typedef struct {
   int autos;       /* 1: append autocorrelation rows */
   int exact_phase; /* 0: fxkernel's stride fringe-phase law (default);
                       1: exact per-sample phase */
-  int chunk_ffts;  /* FFTs per device pass; 0 = choose automatically */
+  int chunk_ffts;  /* FFTs per device pass; 0 = choose automatically: the
+                      whole subintegration while its spectra fit in 32 GiB
+                      of device memory, else as many passes as keep them
+                      there (with overlap: eight passes, at most 4 GiB per
+                      spectra buffer) */
   int overlap;     /* 1: the F-stage of chunk k+1 runs on a second stream
                       while the X-stage of chunk k runs (double-buffered
                       spectra); 0: one stream, one chunk after another */
@@ -399,7 +403,8 @@ typedef struct {
   const gxc_subband *subbands;  /* [nsub], copied by gxc_batch_create */
   int chunks;                   /* device passes per subintegration (each
                                    covers 1/chunks of every subband's FFTs);
-                                   0 = automatic */
+                                   0 = automatic: as few as keep the spectra
+                                   within 32 GiB (eight with overlap) */
   int overlap;                  /* as gxc_config.overlap */
   int graph;                    /* as gxc_config.graph */
   int fstage_mfma;              /* 1: subbands of 2048 points and more get a
