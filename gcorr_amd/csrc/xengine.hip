@@ -17,8 +17,11 @@
 // The transpose happens in registers: a lane loads 8 FFTs (16/8-bit) or 4
 // inputs (FP32) of one 16-byte column, regroups them with v_perm, and
 // stores whole operands (16, 8 or 16 bytes) to LDS.  Chunk positions are
-// XOR-swizzled (16/8-bit) or rows padded (FP32) so that the operand reads
-// of 16 lanes fall in distinct banks.  Rows of inputs past M stay zero.
+// XOR-swizzled by input and by channel (16/8-bit) or rows padded (FP32), so
+// that the operand reads of 16 lanes fall in distinct banks and the staging
+// stores of a half-wave (8 channel columns x 4 inputs) hit each bank at most
+// twice, the least a 16-byte store written as two 8-byte halves of 32 lanes
+// allows.  Rows of inputs past M stay zero.
 //
 // Split partials are [split][channel][plane][a][b]: a wave's stores (and
 // the read-modify-write of later chunks) cover rows of one channel, and the
@@ -58,16 +61,20 @@ struct XM {
   static constexpr int CHUNKB = 8 * EB;
   static constexpr int ROWB = F32 ? MP * 4 + (MB == 1 ? 0 : 64) : TB * EB;
   static constexpr int PLANEB = F32 ? TB * ROWB : MP * ROWB;
-  static constexpr int CHB = 2 * PLANEB + 16;  // +16 B spreads channels over banks
+  // +16 B spreads FP32 channels over banks; the 8-bit layout needs no pad (swc)
+  static constexpr int CHB = 2 * PLANEB + (EB == 1 ? 0 : 16);
   static constexpr int LDSB = CT * CHB;
   // chunk swizzle of the 16/8-bit layout: the 16 inputs of one MFMA block
-  // read their chunk t/8 from 16 distinct bank groups
-  __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : (m >> 3) & 3; }
-  // byte offset of component (plane, input m, FFT t) within a channel;
-  // 16/8-bit: t a multiple of 8 (the start of an operand)
-  __device__ static constexpr int at(int plane, int m, int t) {
+  // read their chunk t/8 from 16 distinct bank groups (swz), and the channel
+  // columns of one staging store spread over the chunks (swc)
+  __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : ((m >> 2) ^ (m >> 4)) & 3; }
+  __device__ static constexpr int swc(int cc) { return EB == 2 ? (cc >> 2) & 3 : (cc >> 3) & 3; }
+  // byte offset of component (plane, input m, FFT t) within channel cc of
+  // the tile (cc only moves the 16/8-bit chunk); 16/8-bit: t a multiple of 8
+  // (the start of an operand)
+  __device__ static constexpr int at(int plane, int m, int t, int cc) {
     return F32 ? plane * PLANEB + t * ROWB + m * 4
-               : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m)) & (TB / 8 - 1)) * CHUNKB;
+               : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
   }
   static_assert(F32 || TB / 8 == 4, "4 chunks per row");
 };
@@ -129,10 +136,10 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
         const uint4 *v = w + 4 * h;
         if (8 * mq + 4 * h >= X::MP) break;  // rows past the padded block count
         const int m0 = 8 * mq + 4 * h;
-        *reinterpret_cast<uint4 *>(d0 + X::at(0, m0, t)) = make_uint4(v[0].x, v[1].x, v[2].x, v[3].x);
-        *reinterpret_cast<uint4 *>(d0 + X::at(1, m0, t)) = make_uint4(v[0].y, v[1].y, v[2].y, v[3].y);
-        *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
-        *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
+        *reinterpret_cast<uint4 *>(d0 + X::at(0, m0, t, 0)) = make_uint4(v[0].x, v[1].x, v[2].x, v[3].x);
+        *reinterpret_cast<uint4 *>(d0 + X::at(1, m0, t, 0)) = make_uint4(v[0].y, v[1].y, v[2].y, v[3].y);
+        *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t, 0)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
+        *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t, 0)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
       }
     }
   } else {
@@ -154,7 +161,6 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
         w[k][2] = v.z;
         w[k][3] = v.w;
       }
-      const int off0 = X::at(0, m, 8 * o), off1 = X::at(1, m, 8 * o);
       if constexpr (X::EB == 2) {
         // word e of a load = (re | im << 16) of channel q*4+e
 #pragma unroll
@@ -165,9 +171,10 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
             re[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x05040100u);
             im[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x07060302u);
           }
-          unsigned char *d = lds + (q * 4 + e) * X::CHB;
-          *reinterpret_cast<uint4 *>(d + off0) = make_uint4(re[0], re[1], re[2], re[3]);
-          *reinterpret_cast<uint4 *>(d + off1) = make_uint4(im[0], im[1], im[2], im[3]);
+          const int cc = q * 4 + e;
+          unsigned char *d = lds + cc * X::CHB;
+          *reinterpret_cast<uint4 *>(d + X::at(0, m, 8 * o, cc)) = make_uint4(re[0], re[1], re[2], re[3]);
+          *reinterpret_cast<uint4 *>(d + X::at(1, m, 8 * o, cc)) = make_uint4(im[0], im[1], im[2], im[3]);
         }
       } else {
         // channel e = bytes 2(e&1) (re), 2(e&1)+1 (im) of word e>>1
@@ -183,9 +190,10 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
           const uint32_t re1 = __builtin_amdgcn_perm(x[3], x[2], 0x05040100u);
           const uint32_t im0 = __builtin_amdgcn_perm(x[1], x[0], 0x07060302u);
           const uint32_t im1 = __builtin_amdgcn_perm(x[3], x[2], 0x07060302u);
-          unsigned char *d = lds + (q * 8 + e) * X::CHB;
-          *reinterpret_cast<uint2 *>(d + off0) = make_uint2(re0, re1);
-          *reinterpret_cast<uint2 *>(d + off1) = make_uint2(im0, im1);
+          const int cc = q * 8 + e;
+          unsigned char *d = lds + cc * X::CHB;
+          *reinterpret_cast<uint2 *>(d + X::at(0, m, 8 * o, cc)) = make_uint2(re0, re1);
+          *reinterpret_cast<uint2 *>(d + X::at(1, m, 8 * o, cc)) = make_uint2(im0, im1);
         }
       }
     }
@@ -224,7 +232,8 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
 #pragma unroll
     for (int w = 0; w < X::CPW; w++) {
-      const unsigned char *ch = lds + (wave + NWV * w) * X::CHB;
+      const int cc = wave + NWV * w;
+      const unsigned char *ch = lds + cc * X::CHB;
 #pragma unroll
       for (int ks = 0; ks < X::TB / X::K; ks++) {
         Op R[MB], I[MB];
@@ -232,8 +241,8 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
         for (int b = 0; b < MB; b++) {
           const int m = b * 16 + (lane & 15);
           const int t = X::F32 ? ks * X::K + (lane >> 4) : (lane >> 4) * 8;
-          R[b] = *reinterpret_cast<const Op *>(ch + X::at(0, m, t));
-          I[b] = *reinterpret_cast<const Op *>(ch + X::at(1, m, t));
+          R[b] = *reinterpret_cast<const Op *>(ch + X::at(0, m, t, cc));
+          I[b] = *reinterpret_cast<const Op *>(ch + X::at(1, m, t, cc));
         }
         int tile = 0;
 #pragma unroll
