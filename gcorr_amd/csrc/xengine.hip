@@ -14,7 +14,7 @@
 // [TB FFTs][M][CT channels] tile through LDS, transposed per channel so
 // that a lane's MFMA operand is contiguous there: [plane][input][t] for the
 // 8-element operands of the 16/8-bit types, [plane][t][input] for FP32.
-// The transpose happens in registers: a lane loads 8 FFTs (16/8-bit) or 4
+// The transpose happens in registers: a lane loads 8 FFTs (16/8-bit) or 8
 // inputs (FP32) of one 16-byte column, regroups them with v_perm, and
 // stores whole operands (16, 8 or 16 bytes) to LDS.  Chunk positions are
 // XOR-swizzled by input and by channel (16/8-bit) or rows padded (FP32), so
