@@ -69,11 +69,16 @@ struct XM {
   // columns of one staging store spread over the chunks (swc)
   __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : ((m >> 2) ^ (m >> 4)) & 3; }
   __device__ static constexpr int swc(int cc) { return EB == 2 ? (cc >> 2) & 3 : (cc >> 3) & 3; }
+  // FP32 at 17 inputs and more: odd FFT rows swap neighbouring 4-input
+  // quads, so that one staging store's lanes cover both 16-byte parities of
+  // the bank groups (X-stage at 16 stations, FP32: 9.73 -> 9.49 ms on
+  // MI355X; at 16 inputs and fewer it measured no better)
+  __device__ static constexpr int swt(int t) { return MB >= 2 ? t & 1 : 0; }
   // byte offset of component (plane, input m, FFT t) within channel cc of
   // the tile (cc only moves the 16/8-bit chunk); 16/8-bit: t a multiple of 8
   // (the start of an operand)
   __device__ static constexpr int at(int plane, int m, int t, int cc) {
-    return F32 ? plane * PLANEB + t * ROWB + m * 4
+    return F32 ? plane * PLANEB + t * ROWB + (((m >> 2) ^ swt(t)) << 4) + (m & 3) * 4
                : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
   }
   static_assert(F32 || TB / 8 == 4, "4 chunks per row");
