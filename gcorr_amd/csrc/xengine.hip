@@ -369,10 +369,18 @@ __device__ __forceinline__ void finalise_channel(const FinMParams &p, int c, flo
   const size_t MM = (size_t)M * M;
   const int tid = threadIdx.x;
   __syncthreads();  // the previous channel's reads of sh are done
-  for (int e = tid; e < 2 * (int)MM; e += 256) {
-    float sum = 0.f, comp = 0.f;
-    for (int s = 0; s < p.splits; s++) neumaier(sum, comp, p.partial[((size_t)s * p.nchan + c) * 2 * MM + e]);
-    sh[e] = sum + comp;
+  if (p.splits == 1) {
+    // one split: a plain copy, unrolled so that a thread's loads are in
+    // flight together
+    const float *src = p.partial + (size_t)c * 2 * MM;
+#pragma unroll 8
+    for (int e = tid; e < 2 * (int)MM; e += 256) sh[e] = src[e];
+  } else {
+    for (int e = tid; e < 2 * (int)MM; e += 256) {
+      float sum = 0.f, comp = 0.f;
+      for (int s = 0; s < p.splits; s++) neumaier(sum, comp, p.partial[((size_t)s * p.nchan + c) * 2 * MM + e]);
+      sh[e] = sum + comp;
+    }
   }
   __syncthreads();
   const float *re_pl = sh, *ir_pl = sh + MM;
