@@ -378,6 +378,7 @@ __device__ __forceinline__ void finalise_channel(const FinMParams &p, int c, flo
   } else {
     for (int e = tid; e < 2 * (int)MM; e += 256) {
       float sum = 0.f, comp = 0.f;
+#pragma unroll 4
       for (int s = 0; s < p.splits; s++) neumaier(sum, comp, p.partial[((size_t)s * p.nchan + c) * 2 * MM + e]);
       sh[e] = sum + comp;
     }
