@@ -50,7 +50,8 @@ class VdifSpec(ctypes.Structure):
         ("frame_bytes", ctypes.c_int), ("thread_id", ctypes.c_int),
         ("frames_per_second", ctypes.c_int), ("ref_epoch", ctypes.c_int),
         ("ref_second", ctypes.c_longlong), ("ref_frame", ctypes.c_int),
-        ("offset_binary8", ctypes.c_int), ("reserved", ctypes.c_int * 2),
+        ("offset_binary8", ctypes.c_int), ("header_bytes", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
 

@@ -95,7 +95,7 @@ class Batch:
 
     def set_input_vdif(self, sub, station, dev_ptr, nbytes, *, frames_per_second, ref_second=0,
                        ref_frame=0, ref_epoch=0, thread_id=0, frame_bytes=0, offset_binary8=True,
-                       keepalive=None, stats=True):
+                       header_bytes=0, keepalive=None, stats=True):
         """Station `station` of subband `sub` from VDIF frames in device memory
         (gxc_batch_set_input_vdif); as FxKernel.setInputVDIF."""
         spec = _lib.VdifSpec()
@@ -104,6 +104,7 @@ class Batch:
         spec.frames_per_second, spec.ref_epoch = int(frames_per_second), int(ref_epoch)
         spec.ref_second, spec.ref_frame = int(ref_second), int(ref_frame)
         spec.offset_binary8 = int(bool(offset_binary8))
+        spec.header_bytes = int(header_bytes)
         st = _lib.VdifStats()
         check_batch(self._L.gxc_batch_set_input_vdif(self._h, int(sub), int(station), ctypes.c_void_p(int(dev_ptr)),
                                                      int(nbytes), ctypes.byref(spec),

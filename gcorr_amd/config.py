@@ -1,4 +1,4 @@
-"""The fxkernel .conf format (fxkernel/src/common.cpp:190-250) via gxc_parse_config."""
+"""The fxkernel .conf format (fxkernel/src/common.cpp:54-114) via gxc_parse_config."""
 import ctypes
 import os
 from dataclasses import dataclass, field

@@ -1,4 +1,4 @@
-// .conf reader: the keyword format of fxkernel/src/common.cpp:190-250.
+// .conf reader: the keyword format of fxkernel/src/common.cpp:54-114.
 //
 // Same keywords (COMPLEX, NBIT, NPOL, NCHAN, LO, BANDWIDTH, NUMFFTS, NANT,
 // case-insensitive), same defaults (NPOL 2, COMPLEX 0, NBIT 2), NANT followed

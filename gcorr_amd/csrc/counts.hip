@@ -6,6 +6,7 @@
 // both stations were valid: fxkernel's baselineCount.
 #include <map>
 #include <mutex>
+#include <set>
 #include <tuple>
 
 #include "gxc_internal.hpp"
@@ -83,6 +84,20 @@ long long persistent_grid(const void *kernel, int block, size_t dyn_lds, long lo
     cache[key] = g;
   }
   return total < g ? total : g;
+}
+
+hipError_t allow_dynamic_lds(const void *kernel, int bytes) {
+  static std::mutex mu;
+  static std::set<std::tuple<int, const void *, int>> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const auto key = std::make_tuple(dev, kernel, bytes);
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count(key)) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) done.insert(key);
+  return e;
 }
 
 hipError_t launch_counts_multi(const CountSeg *segs, int nseg, long long total, hipStream_t s) {

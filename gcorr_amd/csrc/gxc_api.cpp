@@ -280,7 +280,7 @@ extern "C" {
 void gxc_config_init(gxc_config *cfg) {
   if (!cfg) return;
   std::memset(cfg, 0, sizeof(*cfg));
-  cfg->nbit = 2;  // common.cpp:200-202 defaults
+  cfg->nbit = 2;  // common.cpp:64-66 defaults
   cfg->npol = 2;
   cfg->is_complex = 0;
   cfg->precision = GXC_PREC_FP32;
@@ -371,23 +371,39 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   // the whole subint in one pass without overlap (bounded by 32 GiB of the
   // 288 GB of HBM; one pass saves the launch tails of the others: 16
   // stations x 4096 channels, FP16, 15.80 -> 15.34 ms), eight passes with
-  // overlap (bounded by 4 GiB per buffer).
-  h->nbuf = cfg->overlap ? 2 : 1;
+  // overlap (bounded by 4 GiB per buffer).  Either budget is also held to
+  // half of the device memory free at creation, and an automatic chunk that
+  // still does not fit is halved until it does.
   const size_t eb = gxc::xm_element_bytes(out_mode);
   const size_t per_fft = (size_t)d.M * nchan * eb;
+  const bool auto_chunk = cfg->chunk_ffts <= 0;
   long long chunk = cfg->chunk_ffts;
-  if (chunk <= 0) {
-    const size_t budget = cfg->overlap ? (4ull << 30) : (32ull << 30);
+  if (auto_chunk) {
+    size_t budget = cfg->overlap ? (4ull << 30) : (32ull << 30);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+      budget = std::min(budget, free_b / 2 / (cfg->overlap ? 2 : 1));
     chunk = cfg->overlap ? (cfg->numffts + 7) / 8 : cfg->numffts;
     chunk = std::min<long long>(chunk, (long long)(budget / per_fft));
   }
   chunk = std::max<long long>(1, std::min<long long>(chunk, cfg->numffts));
-  if (chunk >= cfg->numffts) h->nbuf = 1;  // one pass: nothing to overlap
-  h->chunk = (int)chunk;
-  for (int b = 0; b < h->nbuf; b++)
-    if ((s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_spec[b]), (size_t)h->chunk * per_fft)) !=
-        GXC_OK)
-      return fail(s);
+  for (;;) {
+    h->chunk = (int)chunk;
+    h->nbuf = (cfg->overlap && chunk < cfg->numffts) ? 2 : 1;  // one pass: nothing to overlap
+    int b = 0;
+    for (; b < h->nbuf; b++)
+      if ((s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_spec[b]), (size_t)h->chunk * per_fft)) !=
+          GXC_OK)
+        break;
+    if (b == h->nbuf) break;
+    if (!auto_chunk || chunk == 1) return fail(s);
+    for (int i = 0; i < b; i++) {
+      hipFree(h->d_spec[i]);
+      h->d_spec[i] = nullptr;
+    }
+    (void)hipGetLastError();  // the failed allocation's error is handled here
+    chunk = (chunk + 1) / 2;
+  }
   // launch shapes: given, or defaults
   {
     int fb[4], nfb = gxc::fstage_blocks(d.L, cfg->is_complex != 0, fb);

@@ -551,14 +551,19 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
       return fail(set_err(b, GXC_ERR_HIP, "hipEventCreate failed"));
 
   // chunks: automatic = 8 passes with overlap, else as few as keep one
-  // spectra buffer within 32 GiB; at most the smallest subband's FFT count,
-  // so every chunk has FFTs of every subband
+  // spectra buffer within 32 GiB and within half of the device memory free
+  // now; at most the smallest subband's FFT count, so every chunk has FFTs
+  // of every subband
   int C = cfg->chunks;
   if (C <= 0) {
     double total = 0;
     for (const Sub &u : b->sub)
       total += (double)u.cfg.numffts * 2 * cfg->nant * u.cfg.nchan * gxc::xm_element_bytes(out_mode);
-    C = cfg->overlap ? 8 : (int)std::ceil(total / (32.0 * (1ull << 30)));
+    double budget = 32.0 * (1ull << 30);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+      budget = std::min(budget, (double)free_b / 2 / (cfg->overlap ? 2 : 1));
+    C = std::max(cfg->overlap ? 8 : 1, (int)std::ceil(total / budget));
   }
   C = std::max(1, std::min(C, minfft));
   b->nchunks = C;

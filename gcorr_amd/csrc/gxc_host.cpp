@@ -64,7 +64,7 @@ gxc_status derive(const gxc_config *c, Derived *d, std::string *why) {
   // fxkernel.cpp:31,37-38
   d->sampletime = 1.0 / (2.0 * c->bw);
   if (c->is_complex) d->sampletime *= 2;
-  const size_t samples = (size_t)(c->numffts + 1) * (size_t)d->L;  // common.cpp:152
+  const size_t samples = (size_t)(c->numffts + 1) * (size_t)d->L;  // common.cpp:16
   d->station_bytes = c->is_complex ? 4 * samples : samples / 2;
   return GXC_OK;
 }

@@ -86,6 +86,10 @@ struct CountSeg {  // units = nrows
 // (occupancy x CU count), at most `total`.
 long long persistent_grid(const void *kernel, int block, size_t dyn_lds, long long total);
 
+// Allow `kernel` dynamic LDS of `bytes` (above the 64 KiB default) on the
+// current device, once per (device, kernel, bytes); safe from any thread.
+hipError_t allow_dynamic_lds(const void *kernel, int bytes);
+
 // Launchers (defined in the .hip files).  Return hipSuccess or the launch error.
 bool fstage_supported(int L, bool cplx);
 // F-stage workgroup sizes available at FFT length L (default first).

@@ -28,6 +28,7 @@ struct VdifParams {
   int thread_id, fps, ref_epoch, ref_frame;
   long long ref_second;
   int nbit, is_complex, flip8;    // flip8: XOR 0x80 (8-bit offset binary)
+  int header_bytes;               // 16 or 32; 0: from the first frame's legacy bit
   unsigned *slot_src;             // [nslots_max] scratch: earliest frame per slot
   long long nslots_max;           // slots for a 32-byte header (the most)
   uint8_t *station;               // [station_bytes], a multiple of 8

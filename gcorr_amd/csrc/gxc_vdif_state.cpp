@@ -48,6 +48,8 @@ gxc_status vdif_prepare(const gxc_config *cfg, size_t station_bytes, const void 
     return *why = "VDIF frames must be 8-byte aligned in device memory", GXC_ERR_INVALID_ARG;
   if (spec->frames_per_second <= 0) return *why = "frames_per_second must be positive", GXC_ERR_INVALID_ARG;
   if (spec->frame_bytes < 0) return *why = "negative frame_bytes", GXC_ERR_INVALID_ARG;
+  if (spec->header_bytes != 0 && spec->header_bytes != 16 && spec->header_bytes != 32)
+    return *why = "header_bytes must be 0, 16 or 32", GXC_ERR_INVALID_ARG;
   if (nbytes < 32) return *why = "VDIF stream shorter than one header", GXC_ERR_INVALID_ARG;
   long long fb = spec->frame_bytes;
   if (fb == 0) {  // frame size from the first header: the one host read of the stream
@@ -56,6 +58,19 @@ gxc_status vdif_prepare(const gxc_config *cfg, size_t station_bytes, const void 
     if (e == hipSuccess) e = hipMemcpy(w, d_frames, sizeof(w), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "reading the first VDIF header", why);
     fb = 8LL * (w[2] & 0xffffffu);
+    // The frame geometry (and, with header_bytes 0, the header size) comes
+    // from this header: reject it here rather than let every frame of the
+    // stream be judged by a malformed first one.
+    const int legacy = (int)((w[0] >> 30) & 1u);
+    if (((w[2] >> 24) & 0x1fu) != 1u || (int)((w[3] >> 26) & 0x1fu) + 1 != cfg->nbit ||
+        (int)(w[3] >> 31) != (cfg->is_complex ? 1 : 0) ||
+        (spec->header_bytes != 0 && legacy != (spec->header_bytes == 16 ? 1 : 0))) {
+      *why = "the first VDIF frame's header does not match the configuration (two channels, " +
+             std::to_string(cfg->nbit) + " bits, " + (cfg->is_complex ? "complex" : "real") +
+             (spec->header_bytes ? ", " + std::to_string(spec->header_bytes) + "-byte header" : std::string()) +
+             ")";
+      return GXC_ERR_INVALID_ARG;
+    }
   }
   // The header size (16 legacy, 32 otherwise) is read from the first frame on
   // the device; buffers are sized for the larger slot count of the two.
@@ -127,6 +142,7 @@ gxc_status vdif_ingest(VdifState &v, const gxc_config *cfg, int station, uint8_t
   vp.nbit = cfg->nbit;
   vp.is_complex = cfg->is_complex ? 1 : 0;
   vp.flip8 = cfg->nbit == 8 && spec->offset_binary8 ? 1 : 0;
+  vp.header_bytes = spec->header_bytes;
   vp.slot_src = v.d_slot_src;
   vp.nslots_max = call.nslots_max;
   vp.station = dst;

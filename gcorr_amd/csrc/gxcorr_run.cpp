@@ -1,8 +1,10 @@
-// gxcorr_run <config> -- the testfxkernel harness (fxkernel/src/testfxkernel.cpp:21-124)
-// on the MI355X path: parse the .conf, read one subintegration of packed data
-// per station, run FxKernel::process once, write vis.out and
-// runtime.fxkernel.log.  Code written against fxkernel compiles against the
-// facade headers unchanged.
+// gxcorr_run <config> [vis.bin] -- the testfxkernel harness
+// (fxkernel/src/testfxkernel.cpp:21-124) on the MI355X path: parse the .conf,
+// read one subintegration of packed data per station, run FxKernel::process
+// once, write vis.out and runtime.fxkernel.log.  Code written against
+// fxkernel compiles against the facade headers unchanged.  The optional
+// second argument also writes the visibilities with their row counts in the
+// binary layout of gxc_save_visibilities_binary.
 #include <chrono>
 #include <ctime>
 #include <fstream>
@@ -20,8 +22,8 @@ using std::string;
 using std::vector;
 
 int main(int argc, char *argv[]) {
-  if (argc != 2) {
-    cout << "Usage:  gxcorr_run <config>\n" << endl;
+  if (argc < 2 || argc > 3) {
+    cout << "Usage:  gxcorr_run <config> [vis.bin]\n" << endl;
     return 1;
   }
   char *configfile = argv[1];
@@ -68,6 +70,7 @@ int main(int argc, char *argv[]) {
     auto t1 = std::chrono::duration_cast<std::chrono::milliseconds>(diff);
     cout << "Run time was " << t1.count() << " milliseconds" << endl;
     fxkernel.saveVisibilities("vis.out");
+    if (argc == 3) fxkernel.saveVisibilitiesBinary(argv[2]);
     std::ofstream ftiming("runtime.fxkernel.log");
     ftiming << "Run time was " << t1.count() << " milliseconds" << endl;
     ftiming << "Start time was " << starttimestring << endl;

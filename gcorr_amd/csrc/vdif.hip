@@ -36,10 +36,11 @@ struct Geom {
   long long nslots;
 };
 
-// Frame geometry from the first frame's legacy bit.
+// Frame geometry: the header size given, else the first frame's legacy bit.
 __device__ __forceinline__ Geom geom(const VdifParams &p) {
   Geom g;
-  g.legacy = (int)((reinterpret_cast<const unsigned *>(p.frames)[0] >> 30) & 1u);
+  g.legacy = p.header_bytes ? (p.header_bytes == 16 ? 1 : 0)
+                            : (int)((reinterpret_cast<const unsigned *>(p.frames)[0] >> 30) & 1u);
   g.hdr = g.legacy ? 16 : 32;
   g.payload = p.frame_bytes - g.hdr;
   g.nslots = ((long long)p.station_bytes + g.payload - 1) / g.payload;

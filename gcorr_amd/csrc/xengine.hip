@@ -506,13 +506,9 @@ hipError_t launch_xm(const XMParams &p, hipStream_t s) {
   const long long units = (long long)(p.nchan / X::CT) * p.splits;
   if (units > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 grid((unsigned)units);
-  static bool attr_set = false;  // dynamic LDS above 64 KiB must be allowed once per kernel
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB, CT, NWV>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  // dynamic LDS above 64 KiB must be allowed once per kernel and device
+  const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void *>(&xmfma_kernel<ET, MB, CT, NWV>), X::LDSB);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((xmfma_kernel<ET, MB, CT, NWV>), grid, dim3(X::NTHR), X::LDSB, s, p);
   return hipGetLastError();
 }
@@ -540,12 +536,8 @@ template <int ET, int MB>
 hipError_t launch_xm_multi(const XSeg *segs, int nseg, long long total, hipStream_t s) {
   using X = XM<ET, MB>;
   const void *fn = reinterpret_cast<const void *>(&xmfma_multi_kernel<ET, MB>);
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, X::LDSB);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  const hipError_t e = allow_dynamic_lds(fn, X::LDSB);
+  if (e != hipSuccess) return e;
   const long long grid = persistent_grid(fn, 256, X::LDSB, total);
   hipLaunchKernelGGL((xmfma_multi_kernel<ET, MB>), dim3((unsigned)grid), dim3(256), X::LDSB, s, segs, nseg,
                      total);

@@ -35,7 +35,7 @@ def _config(nant, nchan, nfft, nbit, lo, bw, is_complex, precision, autos, exact
 
 
 def station_bytes(nant, nchan, nfft, nbit=2, is_complex=False):
-    """allocDataHost's per-station size (fxkernel/src/common.cpp:152)."""
+    """allocDataHost's per-station size (fxkernel/src/common.cpp:16)."""
     c = _config(nant, nchan, nfft, nbit, 0.0, 1.0, is_complex, 0, False, False, 0)
     return int(_lib.load().gxc_station_bytes(ctypes.byref(c)))
 
@@ -133,16 +133,18 @@ class FxKernel:
 
     def setInputVDIF(self, station, dev_ptr, nbytes, *, frames_per_second, ref_second=0,
                      ref_frame=0, ref_epoch=0, thread_id=0, frame_bytes=0, offset_binary8=True,
-                     keepalive=None, stats=True):
+                     header_bytes=0, keepalive=None, stats=True):
         """Station `station`'s input from VDIF frames in device memory
-        (gxc_set_input_vdif).  Returns the ingest counts as a dict (stats=True
-        waits for the ingest)."""
+        (gxc_set_input_vdif).  header_bytes: 32, 16 (legacy) or 0 (from the
+        first frame).  Returns the ingest counts as a dict (stats=True waits
+        for the ingest)."""
         spec = _lib.VdifSpec()
         self._L.gxc_vdif_spec_init(ctypes.byref(spec))
         spec.frame_bytes, spec.thread_id = int(frame_bytes), int(thread_id)
         spec.frames_per_second, spec.ref_epoch = int(frames_per_second), int(ref_epoch)
         spec.ref_second, spec.ref_frame = int(ref_second), int(ref_frame)
         spec.offset_binary8 = int(bool(offset_binary8))
+        spec.header_bytes = int(header_bytes)
         st = _lib.VdifStats()
         check(self._L.gxc_set_input_vdif(self._h, int(station), ctypes.c_void_p(int(dev_ptr)),
                                          int(nbytes), ctypes.byref(spec),

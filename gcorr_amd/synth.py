@@ -65,7 +65,7 @@ class Workload:
 
     @property
     def nsamples(self):
-        """Time samples per station buffer: numffts+1 FFTs (fxkernel/src/common.cpp:152)."""
+        """Time samples per station buffer: numffts+1 FFTs (fxkernel/src/common.cpp:16)."""
         return (self.numffts + 1) * self.fftchannels
 
     @property

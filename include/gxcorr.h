@@ -8,13 +8,13 @@
  *
  * Data layouts (identical to the reference):
  *   input       one packed buffer per station, (numffts+1)*fftchannels time
- *               samples (fxkernel/src/common.cpp:152).  Real 2-bit: 4 samples
+ *               samples (fxkernel/src/common.cpp:16).  Real 2-bit: 4 samples
  *               per byte, LSB first, [A_t, B_t, A_t+1, B_t+1]
  *               (fxkernel/src/fxkernel.cpp:443-476).  Complex 8-bit
  *               (extension): [Are, Aim, Bre, Bim] int8 per time sample.
  *   delays      per station a quadratic d(t)=d[0]*t^2+d[1]*t+d[2] seconds,
  *               t in FFT intervals, plus a file start offset in seconds
- *               (fxkernel/src/fxkernel.h:42-47, common.cpp:219-223).
+ *               (fxkernel/src/fxkernel.h:42-47, common.cpp:77-89).
  *   visibilities [row][product][channel] complex float, rows 0..nbl-1 the
  *               cross baselines (0,1),(0,2)..(0,N-1),(1,2).. and product
  *               p = 2*l+m = X_j,l * conj(X_k,m) (fxkernel.cpp:327-356);
@@ -65,7 +65,7 @@ typedef enum {
 
 /* Correlator configuration: the FxKernel constructor arguments
  * (fxkernel/src/fxkernel.h:29) plus the .conf keywords NPOL
- * (fxkernel/src/common.cpp:230-231) and this build's extensions. */
+ * (fxkernel/src/common.cpp:94-95) and this build's extensions. */
 typedef struct {
   int nant;        /* NANT */
   int nchan;       /* NCHAN, power of two */
@@ -198,7 +198,12 @@ typedef struct {
   long long ref_second;   /* seconds from the epoch of sample 0 */
   int ref_frame;          /* frame number within that second of sample 0 */
   int offset_binary8;     /* 8-bit samples: 1 = offset binary (VDIF), 0 = two's complement */
-  int reserved[2];
+  int header_bytes;       /* 32, or 16 for legacy headers; 0: from the first
+                             frame's legacy bit (then a malformed first frame
+                             makes every frame of another geometry count as
+                             malformed; with frame_bytes 0 the call checks the
+                             first header and rejects a mismatch) */
+  int reserved;
 } gxc_vdif_spec;
 
 typedef struct {
@@ -306,7 +311,7 @@ gxc_status gxc_lag_snr_series(const gxc_cf32 *vis, int nseries, int nchan, float
 /* Device pointer of the visibilities [rows][4][nchan] cf32 (stream-ordered). */
 const void *gxc_visibilities_device(const gxc_handle *h);
 
-/* Bytes per station buffer, fxkernel/src/common.cpp:152 (numffts+1 FFTs). */
+/* Bytes per station buffer, fxkernel/src/common.cpp:16 (numffts+1 FFTs). */
 size_t gxc_station_bytes(const gxc_config *cfg);
 
 /* Visibility rows: nant*(nant-1)/2 cross baselines (+ nant autos). */
@@ -466,7 +471,7 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out);
 gxc_status gxc_batch_set_profiling(gxc_batch *b, int enable);
 gxc_status gxc_batch_get_timings(const gxc_batch *b, gxc_timings *t);
 
-/* .conf reader, fxkernel/src/common.cpp:190-250 (same keywords, same
+/* .conf reader, fxkernel/src/common.cpp:54-114 (same keywords, same
  * defaults, unknown keywords reported and skipped).  The strings and arrays
  * are owned by the struct and released by gxc_conffile_free. */
 typedef struct {

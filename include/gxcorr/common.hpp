@@ -11,7 +11,7 @@
 
 #include "../gxcorr.h"
 
-// fxkernel/src/common.cpp:190-250
+// fxkernel/src/common.cpp:54-114
 inline void parseConfig(char *configfilename, int &nbit, int &nPol, bool &iscomplex, int &nchan, int &nant,
                         double &lo, double &bandwidth, int &numffts, std::vector<std::string> &antenna,
                         std::vector<std::string> &antFiles, double ***delays, double **antfileoffsets) {
@@ -43,7 +43,7 @@ inline void parseConfig(char *configfilename, int &nbit, int &nPol, bool &iscomp
   gxc_conffile_free(&cf);
 }
 
-// fxkernel/src/common.cpp:139-172; station buffers as numffts+1 FFTs.
+// fxkernel/src/common.cpp:3-36; station buffers as numffts+1 FFTs.
 inline void allocDataHost(uint8_t ***data, int numantenna, int numchannels, int numffts, int nbit, int nPol,
                           bool iscomplex, int &subintbytes) {
   gxc_config c;
@@ -73,7 +73,7 @@ inline void allocDataHost(uint8_t ***data, int numantenna, int numchannels, int 
   }
 }
 
-// fxkernel/src/common.cpp:174-188
+// fxkernel/src/common.cpp:38-52
 inline int readdata(int bytestoread, std::vector<std::ifstream *> &antStream, uint8_t **inputdata) {
   for (size_t i = 0; i < antStream.size(); i++) {
     antStream[i]->read((char *)inputdata[i], bytestoread);

@@ -46,7 +46,7 @@ int fxref_derive(const fxref_config *c, fxref_derived *d) {
   d->maxoffset = (c->numffts - 1) * d->fftchannels; /* :272 */
   d->nbaselines = c->nant * (c->nant - 1) / 2;       /* :147 */
   d->nrows = d->nbaselines + (c->autos ? c->nant : 0);
-  /* common.cpp:152 sizes a station buffer as numffts+1 FFTs.  Complex
+  /* common.cpp:16 sizes a station buffer as numffts+1 FFTs.  Complex
    * 8-bit takes 4 bytes per time sample (2 pols x re,im). */
   const size_t samples = (size_t)(c->numffts + 1) * (size_t)d->fftchannels;
   d->bytes_per_station = c->is_complex ? 4 * samples : samples / 2;
@@ -117,7 +117,7 @@ long long fxref_vdif_place(const fxref_config *c, const uint8_t *stream,
   fxref_derived d;
   if (fxref_derive(c, &d) || nbytes < 16 || spec->frames_per_second <= 0) return -1;
   memset(stats, 0, sizeof(*stats));
-  const uint32_t legacy0 = (le32(stream) >> 30) & 1;
+  const uint32_t legacy0 = spec->header_bytes ? (spec->header_bytes == 16 ? 1u : 0u) : (le32(stream) >> 30) & 1;
   const long long hdr = legacy0 ? 16 : 32;
   const long long fb = spec->frame_bytes ? spec->frame_bytes
                                          : 8LL * (le32(stream + 8) & 0xffffff);

@@ -174,6 +174,7 @@ typedef struct {
   long long ref_second;
   int ref_frame;
   int offset_binary8;
+  int header_bytes; /* 16 or 32; 0: from the first frame's legacy bit */
 } fxref_vdif_spec;
 
 typedef struct {

@@ -42,6 +42,7 @@ class VdifSpec(ctypes.Structure):
         ("ref_second", ctypes.c_longlong),
         ("ref_frame", ctypes.c_int),
         ("offset_binary8", ctypes.c_int),
+        ("header_bytes", ctypes.c_int),
     ]
 
 
@@ -250,14 +251,14 @@ def process(cfg, data, delays, offsets, mode="f32"):
 
 
 def vdif_place(cfg, stream, frame_bytes=0, thread_id=0, frames_per_second=1, ref_epoch=0,
-               ref_second=0, ref_frame=0, offset_binary8=True):
+               ref_second=0, ref_frame=0, offset_binary8=True, header_bytes=0):
     """Place a VDIF byte stream into one station buffer (gcorr ingest,
     ingest/src/ingest.c:267-355).  Returns (station bytes, frame_ok flags,
     samples per frame, stats dict)."""
     d = derive(cfg)
     stream = np.ascontiguousarray(stream, np.uint8)
     spec = VdifSpec(int(frame_bytes), int(thread_id), int(frames_per_second), int(ref_epoch),
-                    int(ref_second), int(ref_frame), int(bool(offset_binary8)))
+                    int(ref_second), int(ref_frame), int(bool(offset_binary8)), int(header_bytes))
     station = np.empty(d.bytes_per_station, np.uint8)
     flags = np.empty(d.bytes_per_station // 8 + 1, np.uint8)
     spf = ctypes.c_int(0)

@@ -1,6 +1,6 @@
 """The C-ABI boundary on the CPU: the library loads, exports every function
 include/gxcorr.h declares, validates configurations, and reads the
-reference's .conf fixtures exactly as fxkernel/src/common.cpp:190-250 does.
+reference's .conf fixtures exactly as fxkernel/src/common.cpp:54-114 does.
 No kernel runs here."""
 import ctypes
 import glob
@@ -54,7 +54,7 @@ def test_create_without_device_or_bad_config_fails_cleanly():
 
 
 def test_station_bytes_closed_form():
-    # common.cpp:152: nchan * cfactor * (numffts+1) * nbit/8 * npol
+    # common.cpp:16: nchan * cfactor * (numffts+1) * nbit/8 * npol
     assert gcorr_amd.station_bytes(4, 1024, 3200) == 1024 * 2 * 3201 * 2 // 8 * 2
     assert gcorr_amd.station_bytes(6, 1024, 62500) == 64001024
     # complex 8-bit: 4 bytes per time sample (2 pols x re, im)

@@ -212,3 +212,48 @@ def test_vdif_errors(gpu):
         k.frame_flags(1)
     torch.cuda.synchronize()
     k.close()
+
+
+def test_vdif_header_size_given(gpu):
+    """First frame with a stray legacy bit: header_bytes = 32 keeps the
+    stream's geometry (one malformed frame, visibilities equal plain input);
+    header_bytes = 0 follows the first frame, as the oracle does; a first
+    header that does not match the configuration is rejected when the frame
+    size is read from it."""
+    w, data, poly, offs, rng = case(3, 512, 60, seed=35)
+    fps, payload = 200, 2048
+    cfg = fxref.make_config(w.nant, w.nchan, w.numffts, w.lo, w.bw, autos=True)
+    streams = []
+    for d in data:
+        fr = vdif.frames_from_station(d, payload, frames_per_second=fps)
+        stray = fr[0].copy()
+        vdif.set_header(stray, legacy=1)
+        streams.append(np.concatenate([stray[None], fr]).reshape(-1))
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, autos=True)
+    k.setInputData(data)
+    k.setDelays(poly, offs)
+    k.process()
+    plain = k.visibilities()
+    devs = [torch.from_numpy(s).to("cuda") for s in streams]
+    for hb in (32, 0):
+        for a, dv in enumerate(devs):
+            got = k.setInputVDIF(a, dv.data_ptr(), dv.numel(), frames_per_second=fps,
+                                 frame_bytes=payload + 32, header_bytes=hb)
+            _, ok, _, want = fxref.vdif_place(cfg, streams[a], frames_per_second=fps, header_bytes=hb)
+            assert got == want, (hb, a)
+            np.testing.assert_array_equal(k.frame_flags(a), ok)
+        if hb == 32:
+            k.process()
+            np.testing.assert_array_equal(k.visibilities(), plain)
+    with pytest.raises(gcorr_amd.GxcError):
+        k.setInputVDIF(0, devs[0].data_ptr(), devs[0].numel(), frames_per_second=fps, header_bytes=24)
+    bad = streams[0].copy()
+    vdif.set_header(bad[:payload + 32], bits=8)
+    dbad = torch.from_numpy(bad).to("cuda")
+    with pytest.raises(gcorr_amd.GxcError):  # frame size and geometry from a malformed first header
+        k.setInputVDIF(0, dbad.data_ptr(), dbad.numel(), frames_per_second=fps)
+    got = k.setInputVDIF(0, dbad.data_ptr(), dbad.numel(), frames_per_second=fps, frame_bytes=payload + 32,
+                         header_bytes=32)
+    assert got["malformed"] == 1 and got["missing"] == 0
+    torch.cuda.synchronize()
+    k.close()

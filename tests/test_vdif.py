@@ -185,3 +185,22 @@ def test_missing_frame_rule_closed_form():
         for t in sorted(valid[j] & valid[k]):
             acc += sums[t][0][r]
         np.testing.assert_allclose(v2[r], acc / c2[r], rtol=2e-5, atol=2e-5 * np.abs(acc).max() / c2[r])
+
+
+def test_header_size_given_or_from_first_frame():
+    """A stream whose first frame claims a legacy (16-byte) header: by
+    default the geometry follows that frame and every well-formed frame is
+    malformed against it; with header_bytes = 32 the stray first frame is the
+    only malformed one and the station buffer is whole."""
+    cfg, d = small()
+    buf = np.arange(d.bytes_per_station, dtype=np.uint64).astype(np.uint8)
+    fr = vdif.frames_from_station(buf, 256, frames_per_second=10)
+    stray = fr[0].copy()
+    vdif.set_header(stray, legacy=1)
+    stream = np.concatenate([stray[None], fr]).reshape(-1)
+    n = fr.shape[0]
+    st, ok, _, stats = fxref.vdif_place(cfg, stream, frames_per_second=10, header_bytes=32)
+    np.testing.assert_array_equal(st, buf)
+    assert ok.all() and stats["malformed"] == 1 and stats["placed"] == n
+    _, ok0, _, stats0 = fxref.vdif_place(cfg, stream, frames_per_second=10)
+    assert stats0["malformed"] == n and stats0["placed"] == 1 and ok0.sum() == 1
