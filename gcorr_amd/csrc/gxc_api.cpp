@@ -104,6 +104,7 @@ gxc_status run_xstage(gxc_handle *h, const void *spec, int nt, int accumulate, h
   gxc::XMParams x{};
   x.spectra = spec;
   x.partial = h->d_partial;
+  x.comp = h->d_comp;
   x.nt = nt;
   x.M = h->d.M;
   x.nchan = h->cfg.nchan;
@@ -123,6 +124,7 @@ gxc_status run_finalise(gxc_handle *h, hipStream_t s) {
   GXC_HIP(h, gxc::launch_counts(cp, s));
   gxc::FinMParams f{};
   f.partial = h->d_partial;
+  f.comp = h->d_comp;
   f.scales = h->out_mode == gxc::OUT_F32 ? nullptr : h->d_scales;
   f.rowmap = h->d_rowmap;
   f.counts = h->d_counts;
@@ -425,6 +427,12 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
   if ((s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_partial),
                    gxc::xm_partial_bytes(d.M, nchan, h->splits))) != GXC_OK)
     return fail(s);
+  // Several chunks per subintegration: later chunks add into the partials
+  // with Neumaier compensation, kept in a buffer of the same shape.
+  if (h->chunk < cfg->numffts &&
+      (s = dmalloc(h, reinterpret_cast<unsigned char **>(&h->d_comp),
+                   gxc::xm_partial_bytes(d.M, nchan, h->splits))) != GXC_OK)
+    return fail(s);
   if (out_mode != gxc::OUT_F32) {
     if ((s = dmalloc(h, &h->d_scales, (size_t)d.M * nchan)) != GXC_OK) return fail(s);
     if ((s = dmalloc(h, &h->d_power, (size_t)gxc::kPowerRows * d.M * nchan)) != GXC_OK) return fail(s);
@@ -457,6 +465,7 @@ void gxc_destroy(gxc_handle *h) {
   if (h->cstream) hipStreamDestroy(h->cstream);
   for (void *p : h->d_spec) hipFree(p);
   hipFree(h->d_partial);
+  hipFree(h->d_comp);
   hipFree(h->d_scales);
   hipFree(h->d_power);
   hipFree(h->d_input_owned);

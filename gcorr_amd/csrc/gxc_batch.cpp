@@ -52,6 +52,7 @@ struct Sub {
   int *d_counts = nullptr;
   float2 *d_vis = nullptr;
   float *d_partial = nullptr;
+  float *d_comp = nullptr;  // compensation of d_partial across chunks (several chunks only)
   // polarisation conversion of the visibilities (gxc_batch_set_pol_conversion)
   bool polconv = false;
   float2 *d_polgains = nullptr;  // [nant][2][nchan]
@@ -229,6 +230,7 @@ gxc_status build_tables(gxc_batch *b) {
       gxc::XSeg &x = xseg[(size_t)c * ns + s];
       x.p.spectra = spec;
       x.p.partial = u.d_partial;
+      x.p.comp = u.d_comp;
       x.p.nt = nt;
       x.p.M = b->M;
       x.p.nchan = u.cfg.nchan;
@@ -257,6 +259,7 @@ gxc_status build_tables(gxc_batch *b) {
     cu += b->nrows;
     gxc::FinMParams &f = finseg[s].p;
     f.partial = u.d_partial;
+    f.comp = u.d_comp;
     f.scales = b->out_mode == gxc::OUT_F32 ? nullptr : u.d_scales;
     f.rowmap = b->d_rowmap;
     f.counts = u.d_counts;
@@ -314,10 +317,13 @@ gxc_status assign_splits(gxc_batch *b) {
     u.splits = (int)std::max(1LL, std::min(std::min(sp, maxs), 256LL));
     if (u.splits > u.partial_cap) {
       hipFree(u.d_partial);
-      u.d_partial = nullptr;
+      hipFree(u.d_comp);
+      u.d_partial = u.d_comp = nullptr;
       u.partial_cap = 0;
-      GXB_TRY(dmalloc(b, reinterpret_cast<unsigned char **>(&u.d_partial),
-                      gxc::xm_partial_bytes(b->M, u.cfg.nchan, u.splits)));
+      const size_t nb = gxc::xm_partial_bytes(b->M, u.cfg.nchan, u.splits);
+      GXB_TRY(dmalloc(b, reinterpret_cast<unsigned char **>(&u.d_partial), nb));
+      // several chunks: later chunks add with Neumaier compensation
+      if (b->nchunks > 1) GXB_TRY(dmalloc(b, reinterpret_cast<unsigned char **>(&u.d_comp), nb));
       u.partial_cap = u.splits;
     }
   }
@@ -625,6 +631,7 @@ void gxc_batch_destroy(gxc_batch *b) {
     hipFree(u.d_counts);
     hipFree(u.d_vis);
     hipFree(u.d_partial);
+    hipFree(u.d_comp);
     hipFree(u.d_polgains);
     u.vs.release();
     hipFree(u.d_scales);

@@ -58,6 +58,8 @@ struct gxc_handle {
   int partial_cap = 0;        // splits the partial buffer holds
   void *d_spec[2] = {nullptr, nullptr};  // [chunk][M][nchan] packed complex
   float *d_partial = nullptr;             // [splits][nchan][2][M][M]
+  float *d_comp = nullptr;                // its compensation across chunks
+                                          // (subintegrations of several chunks)
   float *d_scales = nullptr;              // [M][nchan] (reduced precision)
   float *d_power = nullptr;               // [kPowerRows][M][nchan]
   uint8_t *d_valid = nullptr;             // [nant][numffts]

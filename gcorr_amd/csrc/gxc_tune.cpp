@@ -96,12 +96,21 @@ gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out) {
       }
     const int need = *std::max_element(sps.begin(), sps.end());
     if (need > h->partial_cap) {
-      float *p = nullptr;
-      if (hipMalloc((void **)&p, gxc::xm_partial_bytes(h->d.M, h->cfg.nchan, need)) == hipSuccess) {
+      // the partials and, with several chunks, their compensation buffer
+      const size_t nb = gxc::xm_partial_bytes(h->d.M, h->cfg.nchan, need);
+      float *p = nullptr, *pc = nullptr;
+      bool ok = hipMalloc((void **)&p, nb) == hipSuccess;
+      if (ok && h->d_comp) ok = hipMalloc((void **)&pc, nb) == hipSuccess;
+      if (ok) {
         hipFree(h->d_partial);
         h->d_partial = p;
+        if (h->d_comp) {
+          hipFree(h->d_comp);
+          h->d_comp = pc;
+        }
         h->partial_cap = need;
       } else {
+        hipFree(p);
         (void)hipGetLastError();
         sps.erase(std::remove_if(sps.begin(), sps.end(), [&](int v) { return v > h->partial_cap; }), sps.end());
       }

@@ -14,12 +14,15 @@ namespace gxc {
 struct XMParams {
   const void *spectra;  // [nt][M][nchan] complex (2 x 32, 16 or 8 bits)
   float *partial;       // [splits][nchan][2][M][M]
+  float *comp;          // the same shape: compensation of partial across
+                        // chunks (Neumaier), or null (one chunk)
   int nt, M, nchan, splits;
   int accumulate;       // add into partial instead of overwriting
 };
 
 struct FinMParams {
   const float *partial;  // [splits][nchan][2][M][M]
+  const float *comp;     // compensation of partial, or null
   const float *scales;   // [M][nchan], or null for unscaled spectra
   const int2 *rowmap;    // [nrows] station pairs
   const int *counts;     // [nrows]

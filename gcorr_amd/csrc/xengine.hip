@@ -205,6 +205,18 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
   }
 }
 
+// sum + comp += v with Neumaier's compensation (the rounding error of each
+// addition kept in comp).
+__device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
+#pragma clang fp contract(off)
+  const float t = sum + v;
+  if (fabsf(sum) >= fabsf(v))
+    comp += (sum - t) + v;
+  else
+    comp += (v - t) + sum;
+  sum = t;
+}
+
 // Channels group*CT .. group*CT+CT-1, FFT split `split`, on a workgroup of
 // NWV waves, wave w taking channels w, w + NWV, ... (every thread calls it
 // with the same arguments: it contains __syncthreads).  lds holds XM::LDSB
@@ -267,12 +279,31 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
   // D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4.
   // Partials: [split][channel][plane][a][b], so a wave's stores and the
   // read-modify-write of later chunks cover whole rows of one channel.
+  // With a compensation buffer (subintegrations of several chunks) each
+  // later chunk is added with Neumaier's compensation, the first clears it.
   const size_t MM = (size_t)M * M;
+  // one value of this chunk into element e of a partial plane
+  auto put = [&](float *pl, float *cpl, size_t e, float v) {
+    if (!p.accumulate) {
+      pl[e] = v;
+      if (cpl) cpl[e] = 0.f;
+    } else if (cpl) {
+      float sum = pl[e], comp = cpl[e];
+      neumaier(sum, comp, v);
+      pl[e] = sum;
+      cpl[e] = comp;
+    } else {
+      pl[e] += v;
+    }
+  };
 #pragma unroll
   for (int w = 0; w < X::CPW; w++) {
     const int c = c0 + wave + NWV * w;
-    float *pre = p.partial + ((size_t)split * p.nchan + c) * 2 * MM;
+    const size_t off = ((size_t)split * p.nchan + c) * 2 * MM;
+    float *pre = p.partial + off;
     float *pir = pre + MM;
+    float *cre = p.comp ? p.comp + off : nullptr;
+    float *cir = p.comp ? cre + MM : nullptr;
     int tile = 0;
 #pragma unroll
     for (int bi = 0; bi < MB; bi++)
@@ -283,19 +314,11 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
           const int a = bi * 16 + 4 * (lane >> 4) + r;
           const int b = bj * 16 + (lane & 15);
           if (a < M && b < M) {
-            float vre = acc_re[w][tile][r], vir = acc_ir[w][tile][r];
             const size_t ab = (size_t)a * M + b, ba = (size_t)b * M + a;
-            if (p.accumulate) {
-              vre += pre[ab];
-              vir += pir[ab];
-            }
-            pre[ab] = vre;
-            pir[ab] = vir;
-            if (bi != bj) {  // R_bi I_bj^T holds G_IR transposed
-              float vri = acc_ri[w][tile][r];
-              if (p.accumulate) vri += pir[ba];
-              pir[ba] = vri;
-            }
+            put(pre, cre, ab, acc_re[w][tile][r]);
+            put(pir, cir, ab, acc_ir[w][tile][r]);
+            // R_bi I_bj^T holds G_IR transposed
+            if (bi != bj) put(pir, cir, ba, acc_ri[w][tile][r]);
           }
         }
       }
@@ -348,16 +371,6 @@ __global__ void __launch_bounds__(256) xmfma_multi_kernel(const XSeg *segs, int 
   }
 }
 
-__device__ __forceinline__ void neumaier(float &sum, float &comp, float v) {
-#pragma clang fp contract(off)
-  const float t = sum + v;
-  if (fabsf(sum) >= fabsf(v))
-    comp += (sum - t) + v;
-  else
-    comp += (v - t) + sum;
-  sum = t;
-}
-
 // Finalise of channel c on a workgroup of 256 threads: the split partials
 // of the channel (2 M^2 floats each, contiguous) are summed with Neumaier
 // compensation in split order into LDS, then every (row, product) of the
@@ -369,17 +382,30 @@ __device__ __forceinline__ void finalise_channel(const FinMParams &p, int c, flo
   const size_t MM = (size_t)M * M;
   const int tid = threadIdx.x;
   __syncthreads();  // the previous channel's reads of sh are done
-  if (p.splits == 1) {
+  if (p.splits == 1 && !p.comp) {
     // one split: a plain copy, unrolled so that a thread's loads are in
     // flight together
     const float *src = p.partial + (size_t)c * 2 * MM;
 #pragma unroll 8
     for (int e = tid; e < 2 * (int)MM; e += 256) sh[e] = src[e];
-  } else {
+  } else if (!p.comp) {
     for (int e = tid; e < 2 * (int)MM; e += 256) {
       float sum = 0.f, comp = 0.f;
 #pragma unroll 4
       for (int s = 0; s < p.splits; s++) neumaier(sum, comp, p.partial[((size_t)s * p.nchan + c) * 2 * MM + e]);
+      sh[e] = sum + comp;
+    }
+  } else {
+    // with the chunks' compensation terms: each split's error term joins
+    // the running compensation of the sum over splits
+    for (int e = tid; e < 2 * (int)MM; e += 256) {
+      float sum = 0.f, comp = 0.f;
+#pragma unroll 4
+      for (int s = 0; s < p.splits; s++) {
+        const size_t i = ((size_t)s * p.nchan + c) * 2 * MM + e;
+        neumaier(sum, comp, p.partial[i]);
+        comp += p.comp[i];
+      }
       sh[e] = sum + comp;
     }
   }

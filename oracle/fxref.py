@@ -68,6 +68,28 @@ class Derived(ctypes.Structure):
 
 
 _lib = None
+_lib_path = None  # another build of the same source (use_library), or None
+
+
+def use_library(path):
+    """Load the oracle from `path` (a build of the same source with other
+    compiler flags, e.g. the Makefile's `native` target) from now on; None
+    returns to oracle/libfxref.so."""
+    global _lib, _lib_path
+    _lib, _lib_path = None, path
+
+
+def compile_flags(native=False):
+    """The compiler flags of oracle/Makefile's portable build, or of its
+    `native` target."""
+    flags = None
+    with open(os.path.join(_HERE, "Makefile")) as f:
+        for line in f:
+            if line.startswith("CFLAGS ?="):
+                flags = line.split("=", 1)[1].strip()
+    if flags and native:
+        flags = flags.replace("-march=x86-64-v3", "-march=native")
+    return flags
 
 
 def build():
@@ -79,9 +101,10 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = _lib_path or LIB_PATH
+        if _lib_path is None and not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER
         dp, fp, ip, u8p = (P(ctypes.c_double), P(ctypes.c_float),
                            P(ctypes.c_int), P(ctypes.c_uint8))

@@ -87,6 +87,41 @@ def test_full_size_real_2bit(gpu, key):
     assert metrics.eps(v16, v32) < 1e-3
 
 
+def test_full_size_c4_batch(gpu):
+    """The 8-subband batch at its full 0.5 s FFT counts (15 625 / 31 250 FFTs
+    per subband, 1.9 GB of input), default and autotuned launch shapes:
+    properties per subband, FP16 within 1e-3 of FP32, repeatable."""
+    ws = synth.c4_subbands()
+    inputs = [device_input(w, seed=40 + s) for s, w in enumerate(ws)]
+    delays = [synth.make_delays(w) for w in ws]
+
+    def batch(precision, tune):
+        b = gcorr_amd.Batch(ws[0].nant, [(w.nchan, w.numffts, w.lo, w.bw) for w in ws], precision=precision,
+                            autos=True)
+        for s, (bufs, (poly, offs)) in enumerate(zip(inputs, delays)):
+            b.set_input_device(s, [x.data_ptr() for x in bufs], keepalive=bufs)
+            b.set_delays(s, poly, offs)
+        if tune:
+            print(precision, "tuned", b.autotune(), flush=True)
+        b.process()
+        out = [(b.visibilities(s), b.row_counts(s)) for s in range(len(ws))]
+        b.process()
+        for s in range(len(ws)):
+            np.testing.assert_array_equal(b.visibilities(s), out[s][0])  # deterministic
+        b.close()
+        return out
+
+    v32 = batch("fp32", False)
+    for s, w in enumerate(ws):
+        check_properties(w, *v32[s])
+    for precision, tune in (("fp32", True), ("fp16", False), ("fp16", True)):
+        got = batch(precision, tune)
+        for s in range(len(ws)):
+            np.testing.assert_array_equal(got[s][1], v32[s][1])
+            e = metrics.eps(got[s][0], v32[s][0])
+            assert e < (1e-5 if precision == "fp32" else 1e-3), (precision, tune, s, e)
+
+
 def test_full_size_complex8_32_stations(gpu):
     """32 stations x 32768 channels, 15 625 FFTs of complex int8 (65.5 GB of
     input in HBM): FP8 e4m3 and FP16 forms against FP32."""

@@ -51,10 +51,10 @@ def _case(w):
     return _CACHE[w]
 
 
-def _mixed_vs_oracle(w, precision):
+def _mixed_vs_oracle(w, precision, **shape):
     data, poly, offs, ref, rc = _case(w)
     k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex,
-                           precision=precision, autos=True)
+                           precision=precision, autos=True, **shape)
     k.setInputData(data)
     k.setDelays(poly, offs)
     k.process()
@@ -81,6 +81,40 @@ def test_mixed_process_vs_fp64_oracle(gpu, precision):
     assert e < BOUND[precision]
 
 
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+@pytest.mark.parametrize("fblock", [-1, 256, 0])
+def test_c3_shape_every_fstage_vs_fp64_oracle(gpu, precision, fblock):
+    """The 16-station, 4096-channel tones workload's shape (600 of its 31 250
+    FFTs) through each F-stage the autotuner chooses between at 8192 points:
+    the matrix-core kernel (-1), the one-polarisation kernel (256 threads)
+    and the two-polarisation item kernel (default), against the FP64 oracle."""
+    w = synth.WORKLOADS["c3"].scaled(600)
+    v, ref = _mixed_vs_oracle(w, precision, fblock=fblock)
+    e = metrics.eps(v, ref)
+    print(precision, "fblock", fblock, "eps", e, metrics.amp_phase_errors(v, ref))
+    assert e < BOUND[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp8"])
+def test_c3_shape_matrix_core_fstage_every_xtile(gpu, precision):
+    """The matrix-core F-stage with each X-stage channel tile the library
+    offers at 32 inputs (the autotuner's candidates), 600 FFTs at the
+    16-station shape, against the FP64 oracle."""
+    w = synth.WORKLOADS["c3"].scaled(600)
+    ran = []
+    for xtile in (8, 16, 32, 64):
+        for xsplits in (0, 3):
+            try:
+                v, ref = _mixed_vs_oracle(w, precision, fblock=-1, xtile=xtile, xsplits=xsplits)
+            except gcorr_amd.GxcError:
+                continue  # not a tile of this shape
+            e = metrics.eps(v, ref)
+            print(precision, "xtile", xtile, "xsplits", xsplits, "eps", e)
+            assert e < BOUND[precision], (xtile, xsplits)
+            ran.append(xtile)
+    assert len(set(ran)) >= 2, ran
+
+
 def test_mixed_fp16_small_shapes(gpu):
     for key, n in (("c1", 2000), ("c2", 300)):
         w = synth.WORKLOADS[key].scaled(n)
@@ -102,6 +136,22 @@ def test_mixed_chunked_matches_single_pass(gpu):
         out.append(k.visibilities())
         k.close()
     assert metrics.eps(out[1], out[0]) < 1e-5
+
+
+@pytest.mark.parametrize("key,nfft,chunk", [("c5", 24, 3), ("c2", 2000, 16)])
+def test_chunked_fp32_compensated_vs_fp64_oracle(gpu, key, nfft, chunk):
+    """Subintegrations in many chunks (8 passes at the 32-station complex
+    shape, 125 at the 6-station one): each later chunk adds into the split
+    partials with Neumaier compensation, so the error against the FP64
+    oracle stays that of one pass (fxkernel.cpp:327-367 sums every FFT into
+    one FP32 accumulator; gcorr splits the FFT axis, gxkernel.cu:412-424)."""
+    w = synth.WORKLOADS[key].scaled(nfft)
+    v1, ref = _mixed_vs_oracle(w, "fp32")
+    vc, _ = _mixed_vs_oracle(w, "fp32", chunk_ffts=chunk)
+    e1, ec = metrics.eps(v1, ref), metrics.eps(vc, ref)
+    print(key, "one pass eps", e1, "chunked eps", ec)
+    assert ec < 1e-5
+    assert ec <= 1.25 * e1 + 1e-8
 
 
 def test_mixed_is_deterministic(gpu):
