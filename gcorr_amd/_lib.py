@@ -163,6 +163,8 @@ def load():
         "gxc_fstage": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, vp, u8p]),
         "gxc_xengine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, vp]),
+        "gxc_xengine_noalloc": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, vp, vp]),
         "gxc_host_alloc": (vp, [ctypes.c_size_t]),
         "gxc_host_free": (None, [vp]),
         "gxc_integrate_begin": (ctypes.c_int, [H]),

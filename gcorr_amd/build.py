@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["fstage.hip", "fstage_half.hip", "fstage_double.hip", "fstage_big.hip", "fstage_mfma.hip", "fstage_multi.hip", "xengine.hip", "counts.hip", "vdif.hip", "lags.hip", "polconv.hip", "requant.hip", "gxc_api.cpp", "gxc_vdif_api.cpp", "gxc_vdif_state.cpp", "gxc_convert.cpp", "gxc_stream.cpp", "gxc_output.cpp", "gxc_tune.cpp", "gxc_host.cpp", "gxc_batch.cpp", "config.cpp"]
+SOURCES = ["fstage.hip", "fstage_half.hip", "fstage_double.hip", "fstage_big.hip", "fstage_mfma.hip", "fstage_multi.hip", "xengine.hip", "counts.hip", "vdif.hip", "lags.hip", "polconv.hip", "requant.hip", "gxc_noalloc.hip", "gxc_api.cpp", "gxc_vdif_api.cpp", "gxc_vdif_state.cpp", "gxc_convert.cpp", "gxc_stream.cpp", "gxc_output.cpp", "gxc_tune.cpp", "gxc_host.cpp", "gxc_batch.cpp", "config.cpp"]
 
 
 def _run(cmd):

@@ -334,6 +334,22 @@ gxc_status gxc_fstage(gxc_handle *h, int t0, int nfft, gxc_cf32 *spectra,
 gxc_status gxc_xengine(const gxc_cf32 *spectra, int nant, int nchan, int nfft,
                        int autos, int precision, gxc_cf32 *out);
 
+/* gxc_xengine_noalloc: gxc_xengine over device memory, allocating nothing.
+ * It has no counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  d_spectra: device [nfft][2*nant][nchan] cf32; d_vis: device
+ * [rows][4][nchan] cf32; queued on `stream` (a hipStream_t, NULL for the
+ * default stream), the call does not wait.  Its working memory (row map
+ * and counts, the spectra packed to 16/8 bits for the reduced precisions,
+ * the split partials: 8 (2 nant)^2 nchan bytes per split, at least one)
+ * comes from one fixed device array of GXC_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every GPU it runs on until the process ends; shapes
+ * that need more return GXC_ERR_INVALID_ARG.  All calls on one GPU share
+ * that array, so calls that overlap on one GPU race. */
+#define GXC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+gxc_status gxc_xengine_noalloc(const gxc_cf32 *d_spectra, int nant, int nchan, int nfft,
+                               int autos, int precision, gxc_cf32 *d_vis, void *stream);
+
 /* Autotuner: with input and delays set, times the F-stage at each available
  * workgroup size (FFT blocks per workgroup), the X-stage at each channel
  * tile and split count, and whole subintegrations with and without graph
