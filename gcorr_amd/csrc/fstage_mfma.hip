@@ -233,7 +233,11 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   __shared__ __align__(16) uint32_t st1[TWW * L];     // pass-1 twiddles, [t][c][g][i] (FP32: [t][g][c][i])
   uint32_t *const sl = sd + (SPLIT ? G::PS : 0);      // plane 1 (FP32 form)
   __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
-  __shared__ float slev[4];                     // 2-bit levels (fxkernel.cpp:15-28)
+  // 2-bit levels (fxkernel.cpp:15-28): by 4-bit time slot (the nibble holding
+  // both polarisations, A in its low 2 bits) as (level A, level B), and by
+  // 2-bit code
+  __shared__ float2 slev[16];
+  __shared__ float slev4[4];
   __shared__ Split ssplit[kSplitBatch];         // this workgroup's next units' delay splits
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -270,7 +274,11 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     if constexpr (SPLIT) reinterpret_cast<float2 *>(st1)[e] = twf<L>(m);
     else st1[e] = twh<L>(m);
   }
-  if (tid < 4) slev[tid] = tid == 0 ? -kHiMag : tid == 1 ? -1.0f : tid == 2 ? 1.0f : kHiMag;
+  if (tid < 16) {
+    const float lv[4] = {-kHiMag, -1.0f, 1.0f, kHiMag};
+    slev[tid] = make_float2(lv[tid & 3], lv[tid >> 2]);
+    if (tid < 4) slev4[tid] = lv[tid];
+  }
 
   // LDS words of this lane's first access of each kind; the others sit at
   // constant offsets (last-pass input kc = 4 g + j + 16 ks of the butterfly
@@ -316,6 +324,9 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // This lane's input bytes (both polarisations) for every pass-0 tile:
     // window samples n = b + LQ r, r = 4 g + j, two samples to a byte.  All
     // are requested here so that their latency overlaps the table build.
+    // The reduced-precision forms rotate each sample once for both
+    // polarisations: polarisation A's pass 0 replaces the bytes here by
+    // polarisation B's rotated samples (f16 pairs), its pass-0 operands.
     uint32_t raw[K01][4];
     if (valid) {
       gbyte *src = (gbyte *)p.station[st];
@@ -365,48 +376,86 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // (rotator sub[n % S2] * step[n / S2], fxkernel.cpp:553-558; S2 divides
     // LQ, so sub is one entry per column).
     // Each tile's LDS reads are issued one tile ahead of its arithmetic.
-    struct In0 {
-      float2 rsub, step[4];
-      float lev[4];
-      Tw<SPLIT> tw[4];
-    };
-    auto fetch0 = [&](int k) {
-      In0 f;
-      const int bc = 16 * (wave + NW * k) + c16;
-      const int sh = ((offset + bc) & 1) * 4 + 2 * pol;
-      const float2 *stp = stab + S2 + bc / S2 + (LQ / S2) * 4 * g;
-      f.rsub = stab[bc % S2];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        f.lev[j] = slev[(raw[k][j] >> sh) & 3];
-        f.step[j] = stp[(LQ / S2) * j];
-      }
-      fetch_tw(st0, 16 * (bc / RL) + 4 * g, f.tw);
-      return f;
-    };
-    In0 f0 = fetch0(0);
-#pragma unroll
-    for (int k = 0; k < K01; k++) {
-      In0 f1;
-      if (k + 1 < K01) f1 = fetch0(k + 1);
-      uint32_t wh[4], wl[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const float2 r = cmul(f0.rsub, f0.step[j]);
-        pack<SPLIT>(f0.lev[j] * r.x, f0.lev[j] * r.y, wh[j], wl[j]);
-      }
-      Opnd<SPLIT> xo;
-      xo.h = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-      if constexpr (SPLIT) xo.l = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+    // One tile: the DFT of operand xo, the pass-0 twiddles, the LDS store.
+    auto tile0 = [&](int k, const Opnd<SPLIT> &xo, const Tw<SPLIT> *tw) {
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
       dft_op<SPLIT>(d_rh, d_rl, d_ih, d_il, xo, yr, yi);
       uint32_t oh[4], ol[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], f0.tw[i], oh[i], ol[i]);
+      for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], tw[i], oh[i], ol[i]);
       const int a0 = p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)));
       *reinterpret_cast<uint4 *>(&sd[a0]) = make_uint4(oh[0], oh[1], oh[2], oh[3]);
       if constexpr (SPLIT) *reinterpret_cast<uint4 *>(&sl[a0]) = make_uint4(ol[0], ol[1], ol[2], ol[3]);
-      if (k + 1 < K01) f0 = f1;
+    };
+    auto tw0_of = [&](int k, Tw<SPLIT> *tw) {
+      const int bc = 16 * (wave + NW * k) + c16;
+      fetch_tw(st0, 16 * (bc / RL) + 4 * g, tw);
+    };
+    // The FP32 form rotates each polarisation's samples in its own pass 0;
+    // the reduced-precision forms rotate a sample once, in polarisation A's
+    // pass 0, for both, and keep B's operands in raw.
+    if (SPLIT || pol == 0) {
+      // levels of polarisation A and B (reduced precision) or of this
+      // polarisation alone (FP32 form)
+      using Lev = typename std::conditional<SPLIT, float, float2>::type;
+      struct In0 {
+        float2 rsub, step[4];
+        Lev lev[4];
+        Tw<SPLIT> tw[4];
+      };
+      auto fetch0 = [&](int k) {
+        In0 f;
+        const int bc = 16 * (wave + NW * k) + c16;
+        const int sh = ((offset + bc) & 1) * 4 + (SPLIT ? 2 * pol : 0);
+        const float2 *stp = stab + S2 + bc / S2 + (LQ / S2) * 4 * g;
+        f.rsub = stab[bc % S2];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if constexpr (SPLIT)
+            f.lev[j] = slev4[(raw[k][j] >> sh) & 3];
+          else
+            f.lev[j] = slev[(raw[k][j] >> sh) & 15];
+          f.step[j] = stp[(LQ / S2) * j];
+        }
+        tw0_of(k, f.tw);
+        return f;
+      };
+      In0 f0 = fetch0(0);
+#pragma unroll
+      for (int k = 0; k < K01; k++) {
+        In0 f1;
+        if (k + 1 < K01) f1 = fetch0(k + 1);
+        uint32_t wh[4], wl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const float2 r = cmul(f0.rsub, f0.step[j]);
+          if constexpr (SPLIT) {
+            pack<SPLIT>(f0.lev[j] * r.x, f0.lev[j] * r.y, wh[j], wl[j]);
+          } else {
+            pack<SPLIT>(f0.lev[j].x * r.x, f0.lev[j].x * r.y, wh[j], wl[j]);
+            raw[k][j] = hbits(toh2(f0.lev[j].y * r.x, f0.lev[j].y * r.y));
+          }
+        }
+        Opnd<SPLIT> xo;
+        xo.h = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+        if constexpr (SPLIT) xo.l = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+        tile0(k, xo, f0.tw);
+        if (k + 1 < K01) f0 = f1;
+      }
+    } else if constexpr (!SPLIT) {
+      Tw<SPLIT> t0[4];
+      tw0_of(0, t0);
+#pragma unroll
+      for (int k = 0; k < K01; k++) {
+        Tw<SPLIT> t1[4];
+        if (k + 1 < K01) tw0_of(k + 1, t1);
+        Opnd<SPLIT> xo;
+        xo.h = make_uint4(raw[k][0], raw[k][1], raw[k][2], raw[k][3]);
+        tile0(k, xo, t0);
+        if (k + 1 < K01)
+#pragma unroll
+          for (int i = 0; i < 4; i++) t0[i] = t1[i];
+      }
     }
     __syncthreads();
 
@@ -507,7 +556,13 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const float2 fr = cmul(fs, fstep[qb / TS + (256 / TS) * i]);
-        store_bin_sc<OUT, NCHAN>(p, row, qb + 256 * i, cmul(fr, make_float2(yr[i], yi[i])), sc[k][i]);
+        const float2 y = cmul(fr, make_float2(yr[i], yi[i]));
+        if constexpr (OUT == OUT_F16) {
+          // the scaled pair rounded to f16 in one conversion (as store_bin_sc)
+          static_cast<uint32_t *>(p.out)[row * NCHAN + qb + 256 * i] = hbits(toh2(y.x * sc[k][i], y.y * sc[k][i]));
+        } else {
+          store_bin_sc<OUT, NCHAN>(p, row, qb + 256 * i, y, sc[k][i]);
+        }
       }
     }
     if (pol == 0) __syncthreads();  // the next pass 0 rewrites the buffer
