@@ -67,9 +67,24 @@ def parse(argv=None):
     ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (the autotuner runs before warmup)")
+    ap.add_argument("--shape", action="append", default=[], metavar="PREC:FBLOCK,XTILE,XSPLITS",
+                    help="launch shapes for one precision instead of its autotuning (a profiled "
+                         "run then traces only the kernels that are timed), e.g. fp32:256,8,8")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scales HBM counters)")
     return ap.parse_args(argv)
+
+
+def given_shapes(specs):
+    """--shape PREC:FBLOCK,XTILE,XSPLITS -> {prec: {"fblock", "xtile", "xsplits"}}."""
+    out = {}
+    for sp in specs:
+        prec, _, vals = sp.partition(":")
+        v = [int(x) for x in vals.split(",")]
+        if prec not in ERR_BOUND or len(v) != 3:
+            raise SystemExit("--shape wants PREC:FBLOCK,XTILE,XSPLITS, got %r" % sp)
+        out[prec] = {"fblock": v[0], "xtile": v[1], "xsplits": v[2]}
+    return out
 
 
 def compare_list(spec, main):
@@ -225,13 +240,16 @@ def make_kernel(gcorr_amd, ws, gens, precision, args, shape=None, numffts=None):
 
 
 def time_precision(gcorr_amd, torch, ws, gens, precision, args):
-    """Autotune, warm up and time `steps` subintegrations in `precision`;
-    then the per-stage times from a profiled pass.  Returns (result dict,
-    open handle)."""
-    k = make_kernel(gcorr_amd, ws, gens, precision, args)
+    """Autotune (or take the --shape given), warm up and time `steps`
+    subintegrations in `precision`; then the per-stage times from a profiled
+    pass.  Returns (result dict, open handle)."""
+    shape = given_shapes(args.shape).get(precision) if len(ws) == 1 else None
+    k = make_kernel(gcorr_amd, ws, gens, precision, args, shape=shape)
     input_samples = sum(x.input_samples for x in ws)
     tuning = None
-    if not args.no_autotune:
+    if shape:
+        tuning = dict(shape, given=True)
+    elif not args.no_autotune:
         t = time.time()
         tuning = k.autotune()
         tuning["seconds"] = time.time() - t

@@ -88,9 +88,13 @@ __device__ __forceinline__ void butterflies(float2 (&v)[32], int j, const float2
 #pragma unroll
   for (int bf = 0; bf < NB; bf++) {
     const int jj = j + bf * T;
-    const unsigned m = (unsigned)(jj % NS) * (unsigned)(L / (NS * R));
+    // twiddle m*k of the table as a 32-bit byte offset (m*k < L) from the
+    // table's uniform base: one multiply per load, no 64-bit address math
+    const unsigned mb = (unsigned)(jj % NS) * (unsigned)(L / (NS * R)) * (unsigned)sizeof(float2);
+    const unsigned char *twb = reinterpret_cast<const unsigned char *>(tw);
 #pragma unroll
-    for (int k = 1; k < R; k++) v[bf * R + k] = cmul(v[bf * R + k], tw[m * (unsigned)k]);
+    for (int k = 1; k < R; k++)
+      v[bf * R + k] = cmul(v[bf * R + k], *reinterpret_cast<const float2 *>(twb + mb * (unsigned)k));
     dft<R>(&v[bf * R]);
   }
 }
@@ -152,6 +156,24 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
       store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, make_float2(0.f, 0.f));
     return;
   }
+  // This thread's pass-0 input words, requested before the table build so
+  // that their latency overlaps it: complex 8-bit [Are, Aim, Bre, Bim] as
+  // one 32-bit word per sample; 2-bit as the byte holding the sample (byte
+  // layout [A_t, B_t, A_t+1, B_t+1], fxkernel.cpp:443-476).  Pass-0 point k
+  // of thread j is window sample n = j + k*T, packed sample offset + n.
+  gbyte *src = (gbyte *)p.station[st];
+  const int s0 = offset + j;
+  int rawin[32];
+  if constexpr (CPLX) {
+    gint *w = reinterpret_cast<gint *>(src + 4 * (size_t)s0);
+#pragma unroll
+    for (int k = 0; k < 32; k++) rawin[k] = w[(size_t)k * T];
+  } else {
+    // T is even, so every k shares the nibble parity of s0
+    gbyte *bsrc = src + (s0 >> 1);
+#pragma unroll
+    for (int k = 0; k < 32; k++) rawin[k] = bsrc[(size_t)k * (T / 2)];
+  }
   fill_tables<L, CPLX>(p, frac, a, b, stab, j, T);
   // 2-bit levels by code (fxkernel.cpp:15-28): an LDS read replaces the
   // compare/select chain that would otherwise cost ~6 VALU per sample
@@ -168,29 +190,22 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
   // sub[n % S2] * step[n / S2] (fxkernel.cpp:553-558) = sub[j % S2] *
   // step[j / S2 + k*T/S2].
   const float2 rsub = sub[j % S2];
-  gbyte *src = (gbyte *)p.station[st];
-  const int s0 = offset + j;
 
   float2 v[32];
   if constexpr (CPLX) {
-    // [Are, Aim, Bre, Bim] int8 per sample, one little-endian 32-bit word
-    gint *w = reinterpret_cast<gint *>(src + 4 * (size_t)s0);
     const int sh = 16 * pol;
 #pragma unroll
     for (int k = 0; k < 32; k++) {
-      const int wv = w[(size_t)k * T];
+      const int wv = rawin[k];
       const float re = (float)(int8_t)((wv >> sh) & 0xff), im = (float)(int8_t)((wv >> (sh + 8)) & 0xff);
       v[k] = cmul(cmul(rsub, step[j / S2 + k * (T / S2)]), make_float2(re, im));
     }
   } else {
-    // byte layout [A_t, B_t, A_t+1, B_t+1] (fxkernel.cpp:443-476): sample
-    // s is nibble s&1 of byte s>>1; pol A in the nibble's low 2 bits.
-    // T is even, so every k shares the nibble parity of s0.
-    gbyte *bsrc = src + (s0 >> 1);
+    // sample s is nibble s&1 of its byte; pol A in the nibble's low 2 bits
     const int sh = (s0 & 1) * 4 + 2 * pol;
 #pragma unroll
     for (int k = 0; k < 32; k++) {
-      const int code = (bsrc[(size_t)k * (T / 2)] >> sh) & 3;
+      const int code = (rawin[k] >> sh) & 3;
       const float x = lut[code];
       const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
       v[k] = make_float2(x * r.x, x * r.y);
