@@ -80,6 +80,44 @@ __device__ __forceinline__ void get_plane(float2 (&v)[32], const float *s, int j
   }
 }
 
+// v[k] *= w^k for k = 1 .. R-1, w = W_L^m, table entry m of the byte
+// offset mb (entries m*k < L).  Only w^1 .. w^3 and w^(4 * 2^i) come from
+// the table (in L2: at 8192 points the whole table is 64 KiB, and loading
+// every twiddle cost a third of the F-stage); w^(4a) is the product of the
+// loaded powers of the bits of a and w^(4a + b) = w^(4a) w^b, at most three
+// FP32 products (a few ulps) from table entries.
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 *v, const unsigned char *twb, unsigned mb) {
+  auto ld = [&](unsigned k) { return *reinterpret_cast<const float2 *>(twb + mb * k); };
+  float2 p[4];
+#pragma unroll
+  for (int b = 1; b < 4 && b < R; b++) {
+    p[b] = ld((unsigned)b);
+    v[b] = cmul(v[b], p[b]);
+  }
+  if constexpr (R > 4) {
+    constexpr int NQ = ilog2c(R) - 2;  // w^4, w^8, .., w^(R/2)
+    float2 q[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = ld(4u << i);
+#pragma unroll
+    for (int a = 1; 4 * a < R; a++) {
+      float2 w4 = make_float2(1.f, 0.f);
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < NQ; i++)
+        if (a & (1 << i)) {
+          w4 = first ? q[i] : cmul(w4, q[i]);
+          first = false;
+        }
+      v[4 * a] = cmul(v[4 * a], w4);
+#pragma unroll
+      for (int b = 1; b < 4; b++)
+        if (4 * a + b < R) v[4 * a + b] = cmul(v[4 * a + b], cmul(w4, p[b]));
+    }
+  }
+}
+
 // Twiddles and butterflies of pass P >= 1.
 template <int L, bool CPLX, int P>
 __device__ __forceinline__ void butterflies(float2 (&v)[32], int j, const float2 *__restrict__ tw) {
@@ -88,13 +126,9 @@ __device__ __forceinline__ void butterflies(float2 (&v)[32], int j, const float2
 #pragma unroll
   for (int bf = 0; bf < NB; bf++) {
     const int jj = j + bf * T;
-    // twiddle m*k of the table as a 32-bit byte offset (m*k < L) from the
-    // table's uniform base: one multiply per load, no 64-bit address math
+    // table entry m as a 32-bit byte offset from the table's uniform base
     const unsigned mb = (unsigned)(jj % NS) * (unsigned)(L / (NS * R)) * (unsigned)sizeof(float2);
-    const unsigned char *twb = reinterpret_cast<const unsigned char *>(tw);
-#pragma unroll
-    for (int k = 1; k < R; k++)
-      v[bf * R + k] = cmul(v[bf * R + k], *reinterpret_cast<const float2 *>(twb + mb * (unsigned)k));
+    twiddle_powers<R>(&v[bf * R], reinterpret_cast<const unsigned char *>(tw), mb);
     dft<R>(&v[bf * R]);
   }
 }

@@ -153,21 +153,39 @@ constexpr int pt2r(int w, int c, int g, int k, int ks, int j) {
   return MG<L>::S * (16 * (w + MG<L>::NW * k) + c) + (16 * ks + 4 * g + j) / RL + 256 * ((16 * ks + 4 * g + j) % RL);
 }
 
-// W_L^m = exp(-2 pi i m / L) as packed f16
+// W_L^m = exp(-2 pi i m / L) times `scale` as packed f16
 template <int L>
-__device__ __forceinline__ uint32_t twh(int m) {
+__device__ __forceinline__ uint32_t twh(int m, float scale) {
   float s, c;
   sincospif(2.0f * (float)(m % L) / (float)L, &s, &c);
-  return hbits(toh2(c, -s));
+  return hbits(toh2(c * scale, -s * scale));
 }
 
-// One lane's operand of a DFT matrix: row `row` of the 16-row tile, complex
-// K index kc = 16 ks + 4 g + j (j = 0..3) as (re, im) half pairs.  Output
-// row -> butterfly so = row / O, output rp = row % O; input kc -> butterfly
-// kc / R, element kc % R; zero off the diagonal blocks.  hi = f16(w),
-// lo = f16(w - hi).
+// The reduced-precision forms multiply by each DFT-R matrix scaled by a
+// constant chosen so that every entry s cos(2 pi k / R), s sin(2 pi k / R)
+// lies within a small fraction of an f16 ulp of an f16 value: one f16
+// matrix then stands in for the hi + lo pair, and its rounding, the same
+// matrix for every FFT, does not add up coherently over the FFTs of a
+// subintegration on narrowband signals.  Largest relative entry error: R = 8
+// at s = 1.3603517, 1.3e-7; R = 16 at 1.8603300, 1.7e-5; R = 32 at
+// 1.6268249, 8.6e-5 (2.9e-4 .. 3.0e-4 unscaled for R = 16, 32).  The data
+// carry the scales through the passes (at most s16^2 = 3.5 times the f16
+// values of the unscaled transform, far inside the f16 range) and the
+// last pass's FP32 fractional-sample factor divides them out, so that the
+// twiddle tables keep their exact entries (1, -i, ...).
+template <int R>
+constexpr float dft_scale() {
+  return R == 8 ? 1.3603517f : R == 16 ? 1.8603300f : R == 32 ? 1.6268249f : 1.0f;
+}
+
+// One lane's operand of a DFT matrix times `scale`: row `row` of the 16-row
+// tile, complex K index kc = 16 ks + 4 g + j (j = 0..3) as (re, im) half
+// pairs.  Output row -> butterfly so = row / O, output rp = row % O; input
+// kc -> butterfly kc / R, element kc % R; zero off the diagonal blocks.
+// hi = f16(w), lo = f16(w - hi).
 template <int R, int O>
-__device__ void dft_operand(int row, int g, int ks, uint4 &re_hi, uint4 &re_lo, uint4 &im_hi, uint4 &im_lo) {
+__device__ void dft_operand(int row, int g, int ks, float scale, uint4 &re_hi, uint4 &re_lo, uint4 &im_hi,
+                            uint4 &im_lo) {
   const int so = row / O, rp = row % O;
   uint32_t w[4][4];
 #pragma unroll
@@ -175,6 +193,8 @@ __device__ void dft_operand(int row, int g, int ks, uint4 &re_hi, uint4 &re_lo, 
     const int kc = 16 * ks + 4 * g + j;
     float c = 0.f, s = 0.f;
     if (kc / R == so) sincospif(2.0f * (float)(((kc % R) * rp) % R) / (float)R, &s, &c);
+    c *= scale;
+    s *= scale;
     // W = cos - i sin:  Y_re = cos Xr + sin Xi,  Y_im = -sin Xr + cos Xi
     const h2 hr = toh2(c, s), hi = toh2(-s, c);
     const float2v fr = __builtin_convertvector(hr, float2v), fi = __builtin_convertvector(hi, float2v);
@@ -189,22 +209,17 @@ __device__ void dft_operand(int row, int g, int ks, uint4 &re_hi, uint4 &re_lo, 
   im_lo = make_uint4(w[3][0], w[3][1], w[3][2], w[3][3]);
 }
 
-// Y = A . B for one 16-column tile: both output components, A as hi + lo.
-__device__ __forceinline__ void dft_tile(const uint4 &rh, const uint4 &rl, const uint4 &ih, const uint4 &il,
-                                         const uint4 &b, floatx4 &yr, floatx4 &yi) {
-  yr = mma(rh, b, yr);
-  yr = mma(rl, b, yr);
-  yi = mma(ih, b, yi);
-  yi = mma(il, b, yi);
-}
-
-// The same with B as hi + lo for the FP32 form (A_lo B_lo, below 2^-22 of
-// the product, is left out).
+// Y = A . B for one 16-column tile, both output components.  The FP32 form
+// takes A and B as hi + lo (A_lo B_lo, below 2^-22 of the product, left
+// out); the reduced-precision forms one scaled f16 matrix (dft_scale).
 template <bool SPLIT>
 __device__ __forceinline__ void dft_op(const uint4 &rh, const uint4 &rl, const uint4 &ih, const uint4 &il,
                                        const Opnd<SPLIT> &b, floatx4 &yr, floatx4 &yi) {
-  dft_tile(rh, rl, ih, il, b.h, yr, yi);
+  yr = mma(rh, b.h, yr);
+  yi = mma(ih, b.h, yi);
   if constexpr (SPLIT) {
+    yr = mma(rl, b.h, yr);
+    yi = mma(il, b.h, yi);
     yr = mma(rh, b.l, yr);
     yi = mma(ih, b.l, yi);
   }
@@ -244,8 +259,10 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   const int c16 = lane & 15, g = lane >> 4;
 
   // Per-launch constants of this lane: the DFT matrices.
+  // (scaled for the reduced-precision forms, whose lo parts go unused)
+  constexpr float S16 = SPLIT ? 1.0f : dft_scale<16>(), SL = SPLIT ? 1.0f : dft_scale<RL>();
   uint4 d_rh, d_rl, d_ih, d_il;  // DFT-16 of passes 0 and 1
-  dft_operand<16, 16>(c16, g, 0, d_rh, d_rl, d_ih, d_il);
+  dft_operand<16, 16>(c16, g, 0, S16, d_rh, d_rl, d_ih, d_il);
   // Pruned DFT-RL of the last pass.  Its two K-steps need only one operand
   // per lane: for RL <= 16 a row's butterfly lies in K-step row / 8 and its
   // rows are zero in the other, so the lane keeps the K-step of its own row
@@ -253,7 +270,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   // RL = 32 the second K-step's matrix is the first with row r scaled by
   // W_32^(16 r) = (-1)^r.
   uint4 l_rh, l_rl, l_ih, l_il;
-  dft_operand<RL, O>(c16, g, RL == 32 ? 0 : c16 >> 3, l_rh, l_rl, l_ih, l_il);
+  dft_operand<RL, O>(c16, g, RL == 32 ? 0 : c16 >> 3, SL, l_rh, l_rl, l_ih, l_il);
 
   // Twiddle tables, the same for every unit (f16 pairs, widened exactly in
   // twmul; FP32 for the FP32 form).  Pass-0 output row r of the column tile holding column b takes
@@ -266,13 +283,13 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   // ms; the f16 tables lose about 1 % with the same change at 8192 points).
   for (int e = tid; e < 256; e += G::NTHR) {
     if constexpr (SPLIT) reinterpret_cast<float2 *>(st0)[e] = twf<256>((e & 15) * (e >> 4));
-    else st0[e] = twh<256>((e & 15) * (e >> 4));
+    else st0[e] = twh<256>((e & 15) * (e >> 4), 1.0f);
   }
   for (int e = tid; e < L; e += G::NTHR) {
     const int m = SPLIT ? (((e >> 2) & 15) + 16 * (4 * ((e >> 6) & 3) + (e & 3))) * (e >> 8)
                         : (((e >> 4) & 15) + 16 * (e & 15)) * (e >> 8);
     if constexpr (SPLIT) reinterpret_cast<float2 *>(st1)[e] = twf<L>(m);
-    else st1[e] = twh<L>(m);
+    else st1[e] = twh<L>(m, 1.0f);
   }
   if (tid < 16) {
     const float lv[4] = {-kHiMag, -1.0f, 1.0f, kHiMag};
@@ -515,7 +532,9 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // fsub[q % TS] * fstep[q / TS] (fxkernel.cpp:614-625) in FP32, store.
     // TS divides 256 and 16 S NW, so fsub's entry is the lane's for every
     // tile and row.
-    const float2 fs = fsub[qb0 % TS];
+    // with the 1/s of the three scaled matrices
+    constexpr float UNSCALE = 1.0f / (S16 * S16 * SL);
+    const float2 fs = make_float2(fsub[qb0 % TS].x * UNSCALE, fsub[qb0 % TS].y * UNSCALE);
 #pragma unroll
     for (int k = 0; k < K2; k++) {
       Opnd<SPLIT> x[2];  // B operands of K-steps 0 and 1
