@@ -143,6 +143,8 @@ struct MG {
 // pass-1 store i of tile k, last-pass load j of K-step ks of tile k.
 template <int L>
 constexpr int pt0w(int w, int c, int g, int k) { return 16 * (16 * (w + MG<L>::NW * k) + c) + 4 * g; }
+// the same for tile t of 16 columns
+constexpr int pt0t(int t, int c, int g) { return 16 * (16 * t + c) + 4 * g; }
 template <int L>
 constexpr int pt1r(int w, int c, int g, int k, int j) { return 16 * (w + MG<L>::NW * k) + c + MG<L>::LQ * (4 * g + j); }
 template <int L>
@@ -183,9 +185,10 @@ constexpr float dft_scale() {
 // pairs.  Output row -> butterfly so = row / O, output rp = row % O; input
 // kc -> butterfly kc / R, element kc % R; zero off the diagonal blocks.
 // hi = f16(w), lo = f16(w - hi).
+// The row may carry a complex factor fr + i fi (a twiddle of the output).
 template <int R, int O>
 __device__ void dft_operand(int row, int g, int ks, float scale, uint4 &re_hi, uint4 &re_lo, uint4 &im_hi,
-                            uint4 &im_lo) {
+                            uint4 &im_lo, float fr = 1.f, float fi = 0.f) {
   const int so = row / O, rp = row % O;
   uint32_t w[4][4];
 #pragma unroll
@@ -193,8 +196,10 @@ __device__ void dft_operand(int row, int g, int ks, float scale, uint4 &re_hi, u
     const int kc = 16 * ks + 4 * g + j;
     float c = 0.f, s = 0.f;
     if (kc / R == so) sincospif(2.0f * (float)(((kc % R) * rp) % R) / (float)R, &s, &c);
-    c *= scale;
-    s *= scale;
+    // (c - i s)(fr + i fi), then the scale
+    const float cr = c * fr + s * fi, sr = s * fr - c * fi;
+    c = cr * scale;
+    s = sr * scale;
     // W = cos - i sin:  Y_re = cos Xr + sin Xi,  Y_im = -sin Xr + cos Xi
     const h2 hr = toh2(c, s), hi = toh2(-s, c);
     const float2v fr = __builtin_convertvector(hr, float2v), fi = __builtin_convertvector(hi, float2v);
@@ -271,7 +276,24 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   // W_32^(16 r) = (-1)^r.
   uint4 l_rh, l_rl, l_ih, l_il;
   dft_operand<RL, O>(c16, g, RL == 32 ? 0 : c16 >> 3, SL, l_rh, l_rl, l_ih, l_il);
-
+  // At 8192 points the reduced-precision forms fold pass 0's twiddles into
+  // its matrices.  Output row r of the tile holding column b takes
+  // W_256^(r s), s = b / 32: the same for the 16 columns of a tile, so the
+  // twiddle scales row r of the DFT-16 (the lane's row c16 of the MFMA A
+  // operand).  Wave w takes tiles 4w .. 4w+3, i.e. s = 2w and 2w + 1, one
+  // folded matrix each; pass 0 then rounds its products to f16 directly.
+  constexpr bool FOLD0 = !SPLIT && L == 8192;
+  auto tile_of = [&](int w, int k) { return FOLD0 ? K01 * w + k : w + NW * k; };
+  uint4 f_rh[2], f_ih[2];
+  if constexpr (FOLD0) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float sn, cs;
+      sincospif(2.0f * (float)((c16 * (2 * wave + h)) % 256) / 256.0f, &sn, &cs);
+      uint4 unused0, unused1;
+      dft_operand<16, 16>(c16, g, 0, S16, f_rh[h], unused0, f_ih[h], unused1, cs, -sn);
+    }
+  }
   // Twiddle tables, the same for every unit (f16 pairs, widened exactly in
   // twmul; FP32 for the FP32 form).  Pass-0 output row r of the column tile holding column b takes
   // W_L^(r RL (b / RL)) = W_256^(r s), s = b / RL < 16: st0[16 s + r].
@@ -300,7 +322,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   // LDS words of this lane's first access of each kind; the others sit at
   // constant offsets (last-pass input kc = 4 g + j + 16 ks of the butterfly
   // of column C is point S C + kc / RL + 256 (kc % RL)).
-  const int p0w = G::lay_a(pt0w<L>(wave, c16, g, 0));
+  const int p0w = G::lay_a(pt0t(tile_of(wave, 0), c16, g));
   const int p1r = G::lay_a(pt1r<L>(wave, c16, g, 0, 0));
   const int p1w = G::lay_b(pt1w<L>(wave, c16, g, 0, 0));
   const int p2r = G::lay_b(pt2r<L>(wave, c16, g, 0, 0, 0));
@@ -349,7 +371,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
       gbyte *src = (gbyte *)p.station[st];
 #pragma unroll
       for (int k = 0; k < K01; k++) {
-        gbyte *bp = src + ((offset + 16 * (wave + NW * k) + c16) >> 1) + (LQ / 2) * 4 * g;
+        gbyte *bp = src + ((offset + 16 * tile_of(wave, k) + c16) >> 1) + (LQ / 2) * 4 * g;
 #pragma unroll
         for (int j = 0; j < 4; j++) raw[k][j] = bp[(LQ / 2) * j];
       }
@@ -396,17 +418,25 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     // One tile: the DFT of operand xo, the pass-0 twiddles, the LDS store.
     auto tile0 = [&](int k, const Opnd<SPLIT> &xo, const Tw<SPLIT> *tw) {
       floatx4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-      dft_op<SPLIT>(d_rh, d_rl, d_ih, d_il, xo, yr, yi);
       uint32_t oh[4], ol[4];
+      if constexpr (FOLD0) {
+        dft_op<SPLIT>(f_rh[k >> 1], f_rh[k >> 1], f_ih[k >> 1], f_ih[k >> 1], xo, yr, yi);
 #pragma unroll
-      for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], tw[i], oh[i], ol[i]);
-      const int a0 = p0w + (G::lay_a(pt0w<L>(0, 0, 0, k)) - G::lay_a(pt0w<L>(0, 0, 0, 0)));
+        for (int i = 0; i < 4; i++) oh[i] = hbits(toh2(yr[i], yi[i]));
+      } else {
+        dft_op<SPLIT>(d_rh, d_rl, d_ih, d_il, xo, yr, yi);
+#pragma unroll
+        for (int i = 0; i < 4; i++) twist<SPLIT>(yr[i], yi[i], tw[i], oh[i], ol[i]);
+      }
+      const int a0 = p0w + (G::lay_a(pt0t(tile_of(0, k), 0, 0)) - G::lay_a(pt0t(tile_of(0, 0), 0, 0)));
       *reinterpret_cast<uint4 *>(&sd[a0]) = make_uint4(oh[0], oh[1], oh[2], oh[3]);
       if constexpr (SPLIT) *reinterpret_cast<uint4 *>(&sl[a0]) = make_uint4(ol[0], ol[1], ol[2], ol[3]);
     };
     auto tw0_of = [&](int k, Tw<SPLIT> *tw) {
-      const int bc = 16 * (wave + NW * k) + c16;
-      fetch_tw(st0, 16 * (bc / RL) + 4 * g, tw);
+      if constexpr (!FOLD0) {
+        const int bc = 16 * tile_of(wave, k) + c16;
+        fetch_tw(st0, 16 * (bc / RL) + 4 * g, tw);
+      }
     };
     // The FP32 form rotates each polarisation's samples in its own pass 0;
     // the reduced-precision forms rotate a sample once, in polarisation A's
@@ -422,7 +452,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
       };
       auto fetch0 = [&](int k) {
         In0 f;
-        const int bc = 16 * (wave + NW * k) + c16;
+        const int bc = 16 * tile_of(wave, k) + c16;
         const int sh = ((offset + bc) & 1) * 4 + (SPLIT ? 2 * pol : 0);
         const float2 *stp = stab + S2 + bc / S2 + (LQ / S2) * 4 * g;
         f.rsub = stab[bc % S2];
