@@ -6,9 +6,11 @@
 // bound by that.  Here the radix passes run as MFMA products: a radix-R
 // pass over columns b is Y[r'][b] = sum_r W_R^(r r') X[r][b], a real 16x16x32
 // f16 MFMA per 16 columns with the complex data interleaved along K (re, im
-// of 16 points) and the DFT matrix split into two f16 terms (hi + lo, so its
-// own rounding does not count), accumulated in FP32:
+// of 16 points), accumulated in FP32:
 //   Y_re = [cos | sin] . X,  Y_im = [-sin | cos] . X.
+// The reduced-precision forms take one f16 DFT matrix per pass, scaled so
+// that its entries are nearly exact (dft_scale); the FP32 form splits the
+// matrix into two f16 terms (hi + lo) so that its rounding does not count.
 // L = 16 * 16 * RL, three Stockham passes: DFT-16 (one butterfly per MFMA
 // column), DFT-16 again, then DFT-RL pruned to the RL/2 outputs that hold
 // bins below nchan (fxkernel keeps bins [0, nchan) of the 2*nchan-point FFT),
@@ -17,11 +19,14 @@
 // For the reduced-precision forms the data enter every MFMA as f16 and
 // leave it as FP32.  Between passes the FP32 outputs are multiplied by the
 // next pass's twiddle (an f16 pair from an LDS table, widened inside the
-// FMAs) and rounded to f16 once.  The FP32 form carries each value as an f16
+// FMAs) and rounded to f16 once; at 8192 points pass 0's twiddles are
+// folded into its DFT matrices instead.  The FP32 form carries each value as an f16
 // pair plus the f16 pair of its remainder (22 significant bits), multiplies
 // both into the accumulators (A_hi B_hi + A_lo B_hi + A_hi B_lo) and keeps
-// its twiddles in FP32.  The fringe rotation is FP32 with one rounding of
-// the rotated sample; the last pass's outputs keep FP32 for the
+// its twiddles in FP32.  The fringe rotation is FP32, rounded to f16 once
+// and multiplied by the 2-bit level in packed f16 for the reduced-precision
+// forms (exact for the levels +-1), rounded once after the level product in
+// FP32 for the FP32 form; the last pass's outputs keep FP32 for the
 // fractional-sample rotation and the store.
 //
 // A workgroup of 256 threads (512 at L = 8192) loops over units (persistent
@@ -58,7 +63,10 @@ __device__ __forceinline__ h2 hval(uint32_t v) { return __builtin_bit_cast(h2, v
 // y * w in FP32 with w stored as an f16 pair, rounded to f16 once.  Each
 // product is an FMA with the f16 operand widened inside it (v_fma_mix), so
 // w is never converted on its own: re = fma(yr, wr, -(yi wi)),
-// im = fma(yr, wi, yi wr), each inner product an fma with +0.
+// im = fma(yr, wi, yi wr), each inner product an fma with +0.  (The same
+// product in packed f16, three instructions instead of five, rounds three
+// times: at 2048 points it raised the FP16 error against FP64 from 6.8e-4
+// to 8.8e-4 for 1 % of the F-stage's time.)
 __device__ __forceinline__ uint32_t twmul(float yr, float yi, uint32_t w) {
   const h2 v = hval(w);
   const float wr = (float)v.x, wi = (float)v.y;
@@ -254,9 +262,9 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   uint32_t *const sl = sd + (SPLIT ? G::PS : 0);      // plane 1 (FP32 form)
   __shared__ float2 stab[TG::NTAB];             // rotator tables (fill_tables)
   // 2-bit levels (fxkernel.cpp:15-28): by 4-bit time slot (the nibble holding
-  // both polarisations, A in its low 2 bits) as (level A, level B), and by
-  // 2-bit code
-  __shared__ float2 slev[16];
+  // both polarisations, A in its low 2 bits) as the f16 pair (level A,
+  // level B), and by 2-bit code
+  __shared__ uint32_t slevh[16];
   __shared__ float slev4[4];
   __shared__ Split ssplit[kSplitBatch];         // this workgroup's next units' delay splits
 
@@ -315,7 +323,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
   }
   if (tid < 16) {
     const float lv[4] = {-kHiMag, -1.0f, 1.0f, kHiMag};
-    slev[tid] = make_float2(lv[tid & 3], lv[tid >> 2]);
+    slevh[tid] = hbits(toh2(lv[tid & 3], lv[tid >> 2]));
     if (tid < 4) slev4[tid] = lv[tid];
   }
 
@@ -444,7 +452,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
     if (SPLIT || pol == 0) {
       // levels of polarisation A and B (reduced precision) or of this
       // polarisation alone (FP32 form)
-      using Lev = typename std::conditional<SPLIT, float, float2>::type;
+      using Lev = typename std::conditional<SPLIT, float, uint32_t>::type;
       struct In0 {
         float2 rsub, step[4];
         Lev lev[4];
@@ -461,7 +469,7 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
           if constexpr (SPLIT)
             f.lev[j] = slev4[(raw[k][j] >> sh) & 3];
           else
-            f.lev[j] = slev[(raw[k][j] >> sh) & 15];
+            f.lev[j] = slevh[(raw[k][j] >> sh) & 15];
           f.step[j] = stp[(LQ / S2) * j];
         }
         tw0_of(k, f.tw);
@@ -479,8 +487,12 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
           if constexpr (SPLIT) {
             pack<SPLIT>(f0.lev[j] * r.x, f0.lev[j] * r.y, wh[j], wl[j]);
           } else {
-            pack<SPLIT>(f0.lev[j].x * r.x, f0.lev[j].x * r.y, wh[j], wl[j]);
-            raw[k][j] = hbits(toh2(f0.lev[j].y * r.x, f0.lev[j].y * r.y));
+            // the rotator rounded to f16 once, times each polarisation's
+            // f16 level in packed f16 (+-1 exact; 3.3359 is 3.3359375 in
+            // f16, 1.1e-5 high)
+            const h2 rh = toh2(r.x, r.y), lv = hval(f0.lev[j]);
+            wh[j] = hbits(rh * lv.xx);
+            raw[k][j] = hbits(rh * lv.yy);
           }
         }
         Opnd<SPLIT> xo;

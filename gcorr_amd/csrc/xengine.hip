@@ -110,31 +110,72 @@ struct Operand {
       ET == OUT_F32, float, typename std::conditional<ET == OUT_FP8, uint2, uint4>::type>::type;
 };
 
-// Load one stage's [TB FFTs][M][CT] tile (FFTs ts.. of rows < te) and store
-// it transposed into LDS.  Tasks are strided over the 256 threads; in a
-// task the channel column is the fastest index, so neighbouring lanes load
-// neighbouring 16-byte columns of one spectra row.
+// One stage's [TB FFTs][M][CT] tile (FFTs ts.. of rows < te) is loaded and
+// stored transposed into LDS as tasks strided over the workgroup's threads;
+// in a task the channel column is the fastest index, so neighbouring lanes
+// load neighbouring 16-byte columns of one spectra row.
+//   FP32: task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads, then per
+//     channel, plane and 4 inputs one 16-byte store;
+//   16/8-bit: task = (column q, input m, FFTs 8*o..8*o+7): 8 loads, then per
+//     channel and plane one store of the 8 FFTs' components.
 template <int ET, int MB, int CT, int NWV = 4>
-__device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned char *src, size_t rowbytes,
-                                           int M, int c0, int ts, int te, int tid) {
+struct Stage {
   using X = XM<ET, MB, CT, NWV>;
-  constexpr int NQ = X::CT / X::EPL;  // 16-byte columns per row
-  if constexpr (X::F32) {
-    // task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads in flight,
-    // then per channel, plane and 4 inputs one 16-byte store
-    const int nmq = (M + 7) / 8;
-    const int ntask = NQ * nmq * X::TB;
-    for (int task = tid; task < ntask; task += X::NTHR) {
+  static constexpr int NQ = X::CT / X::EPL;  // 16-byte columns per row
+  // tasks of a stage at M inputs, and at most (M = MP)
+  __device__ static int ntask(int M) { return X::F32 ? NQ * ((M + 7) / 8) * X::TB : NQ * M * (X::TB / 8); }
+  static constexpr int MAXTASK = X::F32 ? NQ * (X::MP / 8) * X::TB : NQ * X::MP * (X::TB / 8);
+  // One task per thread and at most 96 accumulator registers per wave: the
+  // next stage's 8 loads (32 registers) can be held while the MFMAs of this
+  // stage run (at 64 inputs the accumulators take 120 and the prefetch
+  // would cost occupancy or spill).
+  static constexpr bool PREFETCH = MAXTASK <= X::NTHR && X::CPW * X::NTILE * 12 <= 96;
+
+  // The loads are unconditional (rows past M and FFTs past te read the
+  // stage's first word instead, and a stage at or past te, or a task < 0,
+  // reads the first word of src) so that none waits inside a branch; bit k
+  // of the returned mask says whether w[k] holds real data, and store()
+  // writes zeros for the others.  Offsets within a stage are 32-bit (at
+  // most TB * MP rows of rowbytes: 512 MiB at 32768 FP32 channels), and the
+  // 8 loads of a task are one row stride apart (FP32: inputs; 16/8-bit:
+  // FFTs).
+  __device__ static unsigned load(uint4 (&w)[8], const unsigned char *src, size_t rowbytes, int M, int c0, int ts,
+                                  int te, int task) {
+    unsigned ok = 0;
+    const unsigned char *base = ts < te ? src + (size_t)ts * M * rowbytes : src;
+    const unsigned rb = (unsigned)rowbytes;
+    unsigned off0, stride;
+    int kmax;  // loads k < kmax are in range
+    if constexpr (X::F32) {
+      const int nmq = (M + 7) / 8;
       const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
-      uint4 w[8];
+      off0 = ((unsigned)t * M + 8 * mq) * rb + (unsigned)(c0 + q * 2) * X::CB;
+      stride = rb;
+      kmax = task >= 0 && ts + t < te ? M - 8 * mq : 0;
+    } else {
+      const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
+      off0 = ((unsigned)(8 * o) * M + m) * rb + (unsigned)(c0 + q * X::EPL) * X::CB;
+      stride = (unsigned)M * rb;
+      kmax = task >= 0 ? te - ts - 8 * o : 0;
+    }
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int m = 8 * mq + k;
-        w[k] = make_uint4(0, 0, 0, 0);
-        if (m < M && ts + t < te)
-          w[k] = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
-                                                  (size_t)(c0 + q * 2) * X::CB);
-      }
+    for (int k = 0; k < 8; k++) {
+      const bool in = k < kmax;
+      w[k] = *reinterpret_cast<const uint4 *>(base + (in ? off0 + k * stride : 0u));
+      ok |= (unsigned)in << k;
+    }
+    return ok;
+  }
+
+  __device__ static void store(uint4 (&w)[8], unsigned ok, unsigned char *lds, int M, int task) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t mk = 0u - ((ok >> k) & 1u);
+      w[k] = make_uint4(w[k].x & mk, w[k].y & mk, w[k].z & mk, w[k].w & mk);
+    }
+    if constexpr (X::F32) {
+      const int nmq = (M + 7) / 8;
+      const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
       unsigned char *d0 = lds + (q * 2) * X::CHB, *d1 = d0 + X::CHB;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
@@ -146,26 +187,8 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
         *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t, 0)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
         *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t, 0)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
       }
-    }
-  } else {
-    // task = (column q, input m, FFTs 8*o..8*o+7): 8 loads, then per
-    // channel and plane one store of the 8 FFTs' components
-    const int ntask = NQ * M * (X::TB / 8);
-    for (int task = tid; task < ntask; task += X::NTHR) {
+    } else {
       const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
-      uint32_t w[8][4];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int t = 8 * o + k;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (ts + t < te)
-          v = *reinterpret_cast<const uint4 *>(src + ((size_t)(ts + t) * M + m) * rowbytes +
-                                               (size_t)(c0 + q * X::EPL) * X::CB);
-        w[k][0] = v.x;
-        w[k][1] = v.y;
-        w[k][2] = v.z;
-        w[k][3] = v.w;
-      }
       if constexpr (X::EB == 2) {
         // word e of a load = (re | im << 16) of channel q*4+e
 #pragma unroll
@@ -173,8 +196,9 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
           uint32_t re[4], im[4];
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            re[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x05040100u);
-            im[j] = __builtin_amdgcn_perm(w[2 * j + 1][e], w[2 * j][e], 0x07060302u);
+            const uint32_t lo = (&w[2 * j].x)[e], hi = (&w[2 * j + 1].x)[e];
+            re[j] = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+            im[j] = __builtin_amdgcn_perm(hi, lo, 0x07060302u);
           }
           const int cc = q * 4 + e;
           unsigned char *d = lds + cc * X::CHB;
@@ -190,7 +214,7 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
                                (uint32_t)(5 + bb) << 24;
           uint32_t x[4];  // [re_2j, re_2j+1, im_2j, im_2j+1]
 #pragma unroll
-          for (int j = 0; j < 4; j++) x[j] = __builtin_amdgcn_perm(w[2 * j + 1][wi], w[2 * j][wi], sel);
+          for (int j = 0; j < 4; j++) x[j] = __builtin_amdgcn_perm((&w[2 * j + 1].x)[wi], (&w[2 * j].x)[wi], sel);
           const uint32_t re0 = __builtin_amdgcn_perm(x[1], x[0], 0x05040100u);
           const uint32_t re1 = __builtin_amdgcn_perm(x[3], x[2], 0x05040100u);
           const uint32_t im0 = __builtin_amdgcn_perm(x[1], x[0], 0x07060302u);
@@ -203,7 +227,7 @@ __device__ __forceinline__ void stage_tile(unsigned char *lds, const unsigned ch
       }
     }
   }
-}
+};
 
 // sum + comp += v with Neumaier's compensation (the rounding error of each
 // addition kept in comp).
@@ -241,10 +265,34 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
 
   const unsigned char *src = static_cast<const unsigned char *>(p.spectra);
   const size_t rowbytes = (size_t)p.nchan * X::CB;
+  using St = Stage<ET, MB, CT, NWV>;
+  const int ntask = St::ntask(M);
+  // With one task per thread the stage after this one is loaded into
+  // registers before this stage's MFMAs, so that its reads are in flight
+  // while they run; otherwise each stage is loaded and stored in turn.
+  // (The prefetch is issued outside any branch, past the last stage too,
+  // where it reads nothing real: a load inside a branch is waited for at
+  // the branch's end.)
+  const int mytask = tid < ntask ? tid : -1;
+  uint4 pre[8];
+  unsigned pre_ok = 0;
+  if constexpr (St::PREFETCH) pre_ok = St::load(pre, src, rowbytes, M, c0, tb, te, mytask);
   for (int ts = tb; ts < te; ts += X::TB) {
     __syncthreads();  // the previous stage's operand reads are done
-    stage_tile<ET, MB, CT, NWV>(lds, src, rowbytes, M, c0, ts, te, tid);
-    __syncthreads();
+    if constexpr (St::PREFETCH) {
+      if (mytask >= 0) St::store(pre, pre_ok, lds, M, mytask);
+      __syncthreads();
+      pre_ok = St::load(pre, src, rowbytes, M, c0, ts + X::TB, te, mytask);
+      // keep the loads ahead of the MFMAs (the scheduler would sink them)
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+      for (int task = tid; task < ntask; task += X::NTHR) {
+        uint4 w[8];
+        const unsigned ok = St::load(w, src, rowbytes, M, c0, ts, te, task);
+        St::store(w, ok, lds, M, task);
+      }
+      __syncthreads();
+    }
     // Wave w: channels w, w+NWV, ...  Lane l's operand: input l%16 of a block
     // and FFTs 8*(l/16).. (16/8-bit) or FFT l/16 of each K-step (FP32).
 #pragma unroll
