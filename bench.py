@@ -67,23 +67,24 @@ def parse(argv=None):
     ap.add_argument("--chunk-ffts", type=int, default=0, help="FFTs per F/X pass (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (the autotuner runs before warmup)")
-    ap.add_argument("--shape", action="append", default=[], metavar="PREC:FBLOCK,XTILE,XSPLITS",
+    ap.add_argument("--shape", action="append", default=[], metavar="PREC:FBLOCK,XTILE,XSPLITS[,GRAPH]",
                     help="launch shapes for one precision instead of its autotuning (a profiled "
-                         "run then traces only the kernels that are timed), e.g. fp32:256,8,8")
+                         "run then traces only the kernels that are timed), e.g. fp32:256,8,8; "
+                         "GRAPH 0 runs eagerly (default 1: graph replay)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scales HBM counters)")
     return ap.parse_args(argv)
 
 
 def given_shapes(specs):
-    """--shape PREC:FBLOCK,XTILE,XSPLITS -> {prec: {"fblock", "xtile", "xsplits"}}."""
+    """--shape PREC:FBLOCK,XTILE,XSPLITS[,GRAPH] -> {prec: {"fblock", "xtile", "xsplits", "graph"}}."""
     out = {}
     for sp in specs:
         prec, _, vals = sp.partition(":")
         v = [int(x) for x in vals.split(",")]
-        if prec not in ERR_BOUND or len(v) != 3:
-            raise SystemExit("--shape wants PREC:FBLOCK,XTILE,XSPLITS, got %r" % sp)
-        out[prec] = {"fblock": v[0], "xtile": v[1], "xsplits": v[2]}
+        if prec not in ERR_BOUND or len(v) not in (3, 4):
+            raise SystemExit("--shape wants PREC:FBLOCK,XTILE,XSPLITS[,GRAPH], got %r" % sp)
+        out[prec] = {"fblock": v[0], "xtile": v[1], "xsplits": v[2], "graph": bool(v[3]) if len(v) == 4 else True}
     return out
 
 

@@ -21,7 +21,7 @@ def test_more_than_one_gpu_exits_with_message():
     assert r.stdout == ""
 
 
-@pytest.mark.parametrize("name", ["traffic_c2_fp32.json", "traffic_c3_fp16.json"])
+@pytest.mark.parametrize("name", ["traffic_c2_fp32.json", "traffic_c3_fp32.json", "traffic_c3_fp16.json"])
 def test_traffic_files_name_their_shape(name):
     with open(os.path.join(ROOT, "profiles", name)) as f:
         t = json.load(f)
@@ -39,3 +39,12 @@ def test_traffic_used_only_for_the_profiled_shape():
     assert bench.traffic_from_profiles("c2", "fp32", fb + 1, float(lps)) is None
     assert bench.traffic_from_profiles("c2", "fp32", fb, 2.0 * lps) is None
     assert bench.traffic_from_profiles("c1", "fp32", fb, float(lps)) is None
+
+
+def test_given_shapes():
+    got = bench.given_shapes(["fp32:256,16,8", "fp16:-1,32,8,0"])
+    assert got["fp32"] == {"fblock": 256, "xtile": 16, "xsplits": 8, "graph": True}
+    assert got["fp16"] == {"fblock": -1, "xtile": 32, "xsplits": 8, "graph": False}
+    for bad in ("fp32:256,16", "fp32:1,2,3,4,5", "fp64:1,2,3"):
+        with pytest.raises(SystemExit):
+            bench.given_shapes([bad])
