@@ -414,7 +414,7 @@ gxc_status gxc_create(const gxc_config *cfg, gxc_handle **out) {
                gxc::fstage_mfma_supported(d.L, cfg->is_complex != 0, out_mode == gxc::OUT_F32));
     for (int i = 0; i < nfb; i++) ok = ok || cfg->fblock == fb[i];
     if (!ok) return fail(set_err(h, GXC_ERR_INVALID_ARG, "fblock is not an F-stage workgroup size for this FFT length"));
-    int xt[3], nxt = gxc::xm_tiles(d.M, out_mode, xt);
+    int xt[4], nxt = gxc::xm_tiles(d.M, out_mode, xt);
     ok = cfg->xtile == 0;
     for (int i = 0; i < nxt; i++) ok = ok || (cfg->xtile == xt[i] && nchan % xt[i] == 0);
     if (!ok) return fail(set_err(h, GXC_ERR_INVALID_ARG, "xtile is not an X-stage channel tile for this shape"));

@@ -115,7 +115,7 @@ gxc_status gxc_autotune(gxc_handle *h, gxc_tuning *out) {
         sps.erase(std::remove_if(sps.begin(), sps.end(), [&](int v) { return v > h->partial_cap; }), sps.end());
       }
     }
-    int xt[3], nxt = gxc::xm_tiles(h->d.M, h->out_mode, xt);
+    int xt[4], nxt = gxc::xm_tiles(h->d.M, h->out_mode, xt);
     for (int i = 0; i < nxt && st == GXC_OK; i++) {
       if (h->cfg.nchan % xt[i]) continue;
       h->xtile = xt[i];

@@ -56,13 +56,23 @@ struct XM {
   static constexpr int NTHR = 64 * NWV;
   static constexpr int CPW = CT / NWV;  // channels per wave
   // 16/8-bit: per plane, rows m of TB*EB bytes = TB/8 chunks of 8 FFTs.
-  // FP32: per plane, rows t of ROWB bytes holding the inputs contiguously,
-  // padded so that 4 consecutive rows start 64 or 192 bytes apart mod 256.
+  // FP32: per plane, rows t holding the inputs contiguously in 16-byte
+  // chunks of 4 inputs, placed so that the 64-byte block segments of 4
+  // consecutive rows, which one MFMA operand read covers, fall in 4
+  // distinct bank groups: 64-byte rows (16 inputs) do so as they are;
+  // 128-byte rows (32 inputs) swap their block halves in every other pair of
+  // rows (swr); 64 inputs pad each row by 64 bytes.  (The 32-input form
+  // replaced rows padded to 192 bytes: 33 % less LDS, 2 workgroups of the
+  // 16-channel tile per CU instead of 1, C3 FP32 X-stage 7.4 -> 7.1 ms on
+  // MI355X; the same swizzle at 64 inputs measured 2.6 % slower than the
+  // padding.)
   static constexpr int CHUNKB = 8 * EB;
-  static constexpr int ROWB = F32 ? MP * 4 + (MB == 1 ? 0 : 64) : TB * EB;
+  static constexpr int ROWB = F32 ? MP * 4 + (MB == 4 ? 64 : 0) : TB * EB;
   static constexpr int PLANEB = F32 ? TB * ROWB : MP * ROWB;
-  // +16 B spreads FP32 channels over banks; the 8-bit layout needs no pad (swc)
-  static constexpr int CHB = 2 * PLANEB + (EB == 1 ? 0 : 16);
+  // Channel pad: +16 B spreads FP32 channels over banks, +32 B at 32 inputs
+  // (a 64-bank model of one staging store's lanes: a third of the conflicts
+  // of +16 at 8 channels per tile); the 8-bit layout needs no pad (swc).
+  static constexpr int CHB = 2 * PLANEB + (F32 && MB == 2 ? 32 : EB == 1 ? 0 : 16);
   static constexpr int LDSB = CT * CHB;
   // chunk swizzle of the 16/8-bit layout: the 16 inputs of one MFMA block
   // read their chunk t/8 from 16 distinct bank groups (swz), and the channel
@@ -74,11 +84,15 @@ struct XM {
   // the bank groups (X-stage at 16 stations, FP32: 9.73 -> 9.49 ms on
   // MI355X; at 16 inputs and fewer it measured no better)
   __device__ static constexpr int swt(int t) { return MB >= 2 ? t & 1 : 0; }
+  // FP32 row swizzle of the 16-byte chunks at 32 inputs: rows t and t+2 of
+  // 128 bytes share a bank half, so every other pair of rows swaps its two
+  // block halves.
+  __device__ static constexpr int swr(int t) { return MB == 2 ? ((t >> 1) & 1) << 2 : 0; }
   // byte offset of component (plane, input m, FFT t) within channel cc of
   // the tile (cc only moves the 16/8-bit chunk); 16/8-bit: t a multiple of 8
   // (the start of an operand)
   __device__ static constexpr int at(int plane, int m, int t, int cc) {
-    return F32 ? plane * PLANEB + t * ROWB + (((m >> 2) ^ swt(t)) << 4) + (m & 3) * 4
+    return F32 ? plane * PLANEB + t * ROWB + (((m >> 2) ^ swt(t) ^ swr(t)) << 4) + (m & 3) * 4
                : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
   }
   static_assert(F32 || TB / 8 == 4, "4 chunks per row");
