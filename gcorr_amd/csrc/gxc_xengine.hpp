@@ -10,7 +10,9 @@ namespace gxc {
 
 // Partial Gram sums of one split: [nchan][2][M][M] floats, plane 0 the real
 // part Re(V) (upper block triangle), plane 1 G_IR[a][b] = sum Im(x_a) Re(x_b)
-// (all entries), so Im V_ab = G_IR[a][b] - G_IR[b][a].
+// (all entries), so Im V_ab = G_IR[a][b] - G_IR[b][a].  FP32 blocks off the
+// diagonal hold Im V_ab itself at [a][b] (a's block before b's) and 0 at
+// [b][a], which the same difference returns.
 struct XMParams {
   const void *spectra;  // [nt][M][nchan] complex (2 x 32, 16 or 8 bits)
   float *partial;       // [splits][nchan][2][M][M]

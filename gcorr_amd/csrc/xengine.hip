@@ -7,7 +7,12 @@
 //   Re V = R R^T + I I^T,  Im V_ab = G_IR[a][b] - G_IR[b][a],  G_IR = I R^T
 // where R, I are the [M x T] real and imaginary parts.  Each product is a
 // 16x16 MFMA over 16-input blocks: v_mfma_f32_16x16x4f32 (K = 4 FFTs) for
-// FP32, v_mfma_f32_16x16x32_{f16,bf16,fp8_fp8} (K = 32) otherwise.
+// FP32, v_mfma_f32_16x16x32_{f16,bf16,fp8_fp8} (K = 32) otherwise.  FP32
+// blocks off the diagonal take three products instead of four (Gauss):
+// P1 = R_i R_j^T, P2 = I_i I_j^T, P3 = (R_i + I_i)(R_j - I_j)^T, with
+// Re V = P1 + P2 and Im V = P3 - P1 + P2 (the packed forms are read from
+// HBM at full speed with the MFMAs mostly idle, and their sums R + I would
+// add a rounding).
 //
 // The F-stage writes spectra channel-contiguous ([t][input][chan]), the
 // layout its last FFT pass stores coalesced.  Each workgroup stages a
@@ -329,10 +334,19 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
 #pragma unroll
           for (int bj = bi; bj < MB; bj++, tile++) {
             const std::integral_constant<int, ET> et;
-            acc_re[w][tile] = mfma(R[bi], R[bj], acc_re[w][tile], et);
-            acc_re[w][tile] = mfma(I[bi], I[bj], acc_re[w][tile], et);
-            acc_ir[w][tile] = mfma(I[bi], R[bj], acc_ir[w][tile], et);
-            if (bi != bj) acc_ri[w][tile] = mfma(R[bi], I[bj], acc_ri[w][tile], et);
+            if (X::F32 && bi != bj) {
+              // FP32 off the diagonal, three products (Gauss):
+              // P1 = R_i R_j^T, P2 = I_i I_j^T, P3 = (R_i + I_i)(R_j - I_j)^T,
+              // so that Re V = P1 + P2 and Im V = P3 - P1 + P2.
+              acc_re[w][tile] = mfma(R[bi], R[bj], acc_re[w][tile], et);
+              acc_ri[w][tile] = mfma(I[bi], I[bj], acc_ri[w][tile], et);
+              acc_ir[w][tile] = mfma(R[bi] + I[bi], R[bj] - I[bj], acc_ir[w][tile], et);
+            } else {
+              acc_re[w][tile] = mfma(R[bi], R[bj], acc_re[w][tile], et);
+              acc_re[w][tile] = mfma(I[bi], I[bj], acc_re[w][tile], et);
+              acc_ir[w][tile] = mfma(I[bi], R[bj], acc_ir[w][tile], et);
+              if (bi != bj) acc_ri[w][tile] = mfma(R[bi], I[bj], acc_ri[w][tile], et);
+            }
           }
       }
     }
@@ -377,10 +391,19 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
           const int b = bj * 16 + (lane & 15);
           if (a < M && b < M) {
             const size_t ab = (size_t)a * M + b, ba = (size_t)b * M + a;
-            put(pre, cre, ab, acc_re[w][tile][r]);
-            put(pir, cir, ab, acc_ir[w][tile][r]);
-            // R_bi I_bj^T holds G_IR transposed
-            if (bi != bj) put(pir, cir, ba, acc_ri[w][tile][r]);
+            if (X::F32 && bi != bj) {
+              // Im V_ab itself at [a][b] and 0 at [b][a]: the finalise's
+              // G_IR[a][b] - G_IR[b][a] gives it back
+              const float p1 = acc_re[w][tile][r], p2 = acc_ri[w][tile][r], p3 = acc_ir[w][tile][r];
+              put(pre, cre, ab, p1 + p2);
+              put(pir, cir, ab, (p3 - p1) + p2);
+              put(pir, cir, ba, 0.f);
+            } else {
+              put(pre, cre, ab, acc_re[w][tile][r]);
+              put(pir, cir, ab, acc_ir[w][tile][r]);
+              // R_bi I_bj^T holds G_IR transposed
+              if (bi != bj) put(pir, cir, ba, acc_ri[w][tile][r]);
+            }
           }
         }
       }
