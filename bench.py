@@ -323,6 +323,9 @@ def time_precision(gcorr_amd, torch, ws, gens, precision, args):
         "traffic": (traffic or {}).get(dom),
         "algorithmic_bytes_per_launch": stages[dom]["bytes_per_launch"],
     }
+    # what binds the profiled kernel, from its committed counters
+    if traffic and traffic.get("limit", {}).get(dom):
+        roofline["limit"] = traffic["limit"][dom]
     res = {"precision": precision, "ms_per_step": ms_per_step,
            "value": input_samples / (ms_per_step * 1e-3) / 1e9,
            "roofline": roofline, "stages": stages, "tuning": tuning}
