@@ -179,3 +179,24 @@ def test_c5_shape_vs_fp64_oracle(gpu, precision):
     e = metrics.eps(v, ref)
     print(precision, "eps", e, metrics.amp_phase_errors(v, ref))
     assert e < {"fp32": 1e-5, "fp16": 1e-3, "fp8": BOUND["fp8"]}[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_c5_shape_every_xtile(gpu, precision):
+    """64 inputs (4 blocks of 16, 10 block tiles per channel): every X-stage
+    channel tile the library offers there (FP32: 4 channels; FP16: 4 and 8),
+    with one and with several FFT splits, against the FP64 oracle (24 FFTs
+    of the 32-station shape)."""
+    w = synth.WORKLOADS["c5"].scaled(24)
+    ran = []
+    for xtile in (2, 4, 8, 16):
+        for xsplits in (1, 3):
+            try:
+                v, ref = _mixed_vs_oracle(w, precision, xtile=xtile, xsplits=xsplits)
+            except gcorr_amd.GxcError:
+                continue  # not a tile of this shape
+            e = metrics.eps(v, ref)
+            print(precision, "xtile", xtile, "xsplits", xsplits, "eps", e)
+            assert e < {"fp32": 1e-5, "fp16": 1e-3}[precision], (xtile, xsplits)
+            ran.append(xtile)
+    assert len(set(ran)) >= (1 if precision == "fp32" else 2), ran

@@ -206,7 +206,7 @@ def test_process_is_deterministic_and_reusable(gpu):
 # ------------------------------------------------------------------ X-engine KAT
 
 @pytest.mark.parametrize("nant,nchan,nfft", [(3, 256, 4), (5, 1024, 100), (3, 512, 3000),
-                                             (16, 128, 40)])
+                                             (16, 128, 40), (9, 64, 50), (32, 64, 40)])
 def test_xengine_integer_kat(gpu, nant, nchan, nfft):
     """gcorr/test/unit.cc:73-159: integer spectra in [-5, 5] make every sum
     exact, so the GPU result equals the exact sum / nfft bit for bit."""
