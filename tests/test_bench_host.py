@@ -48,3 +48,14 @@ def test_given_shapes():
     for bad in ("fp32:256,16", "fp32:1,2,3,4,5", "fp64:1,2,3"):
         with pytest.raises(SystemExit):
             bench.given_shapes([bad])
+
+
+@pytest.mark.parametrize("name", ["traffic_c3_fp32.json", "traffic_c3_fp16.json"])
+def test_traffic_limit_cites_committed_counters(name):
+    """The roofline's 'limit' note comes with the traffic figures and names
+    the committed counter file it is read from."""
+    with open(os.path.join(ROOT, "profiles", name)) as f:
+        t = json.load(f)
+    note = t["limit"]["fstage"]
+    cited = [w.strip("()") for w in note.split() if w.strip("()").startswith("profiles/")]
+    assert cited and all(os.path.exists(os.path.join(ROOT, c)) for c in cited), note
