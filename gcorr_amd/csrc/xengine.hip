@@ -20,13 +20,12 @@
 // that a lane's MFMA operand is contiguous there: [plane][input][t] for the
 // 8-element operands of the 16/8-bit types, [plane][t][input] for FP32.
 // The transpose happens in registers: a lane loads 8 FFTs (16/8-bit) or 8
-// inputs (FP32) of one 16-byte column, regroups them with v_perm, and
-// stores whole operands (16, 8 or 16 bytes) to LDS.  Chunk positions are
-// XOR-swizzled by input and by channel (16/8-bit) or rows padded (FP32), so
-// that the operand reads of 16 lanes fall in distinct banks and the staging
-// stores of a half-wave (8 channel columns x 4 inputs) hit each bank at most
-// twice, the least a 16-byte store written as two 8-byte halves of 32 lanes
-// allows.  Rows of inputs past M stay zero.
+// inputs (FP32) of one 16-byte column, regroups them with v_perm and stores
+// whole operands (16 or 8 bytes) to LDS, or (FP32) stores each word on its
+// own.  Positions are XOR-swizzled by input, channel and FFT, so that the
+// operand reads of 16 lanes fall in distinct banks and the staging stores
+// spread over the banks (FP32: both conflict-free).  Rows of inputs past M
+// stay zero.
 //
 // Split partials are [split][channel][plane][a][b]: a wave's stores (and
 // the read-modify-write of later chunks) cover rows of one channel, and the
@@ -61,43 +60,39 @@ struct XM {
   static constexpr int NTHR = 64 * NWV;
   static constexpr int CPW = CT / NWV;  // channels per wave
   // 16/8-bit: per plane, rows m of TB*EB bytes = TB/8 chunks of 8 FFTs.
-  // FP32: per plane, rows t holding the inputs contiguously in 16-byte
-  // chunks of 4 inputs, placed so that the 64-byte block segments of 4
-  // consecutive rows, which one MFMA operand read covers, fall in 4
-  // distinct bank groups: 64-byte rows (16 inputs) do so as they are;
-  // 128-byte rows (32 inputs) swap their block halves in every other pair of
-  // rows (swr); 64 inputs pad each row by 64 bytes.  (The 32-input form
-  // replaced rows padded to 192 bytes: 33 % less LDS, 2 workgroups of the
-  // 16-channel tile per CU instead of 1, C3 FP32 X-stage 7.4 -> 7.1 ms on
-  // MI355X; the same swizzle at 64 inputs measured 2.6 % slower than the
-  // padding.)
+  // FP32: per plane, rows t of the MP inputs (64 inputs: rows padded by 16
+  // words), input m of row t of channel cc in word m ^ swf(t) ^ swq(t, cc),
+  // so that one 32-lane half of an MFMA operand read (16 inputs of rows t
+  // and t+1) and one half of a staging store (one word of each lane: 8
+  // channel pairs x 4 eight-input groups) both cover the 32 banks once.
+  // LDS serves 32 lanes x 4 bytes per cycle over 32 banks, and a 16-byte
+  // store as 4 such word passes, so 16-byte stores of aligned chunks hit
+  // every bank at least 4 times: the FP32 staging stores whole words.  (At
+  // 16 stations, FP32, this layout replaced 16-byte stores of 4-input
+  // chunks, which cost 7 extra cycles per word pass on MI355X.)
   static constexpr int CHUNKB = 8 * EB;
   static constexpr int ROWB = F32 ? MP * 4 + (MB == 4 ? 64 : 0) : TB * EB;
   static constexpr int PLANEB = F32 ? TB * ROWB : MP * ROWB;
-  // Channel pad: +16 B spreads FP32 channels over banks, +32 B at 32 inputs
-  // (a 64-bank model of one staging store's lanes: a third of the conflicts
-  // of +16 at 8 channels per tile); the 8-bit layout needs no pad (swc).
-  static constexpr int CHB = 2 * PLANEB + (F32 && MB == 2 ? 32 : EB == 1 ? 0 : 16);
+  // Channel pad: +16 B spreads the 16-bit channels over banks; FP32 and
+  // the 8-bit layout need none (swq, swc).
+  static constexpr int CHB = 2 * PLANEB + (F32 || EB == 1 ? 0 : 16);
   static constexpr int LDSB = CT * CHB;
   // chunk swizzle of the 16/8-bit layout: the 16 inputs of one MFMA block
   // read their chunk t/8 from 16 distinct bank groups (swz), and the channel
   // columns of one staging store spread over the chunks (swc)
   __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : ((m >> 2) ^ (m >> 4)) & 3; }
   __device__ static constexpr int swc(int cc) { return EB == 2 ? (cc >> 2) & 3 : (cc >> 3) & 3; }
-  // FP32 at 17 inputs and more: odd FFT rows swap neighbouring 4-input
-  // quads, so that one staging store's lanes cover both 16-byte parities of
-  // the bank groups (X-stage at 16 stations, FP32: 9.73 -> 9.49 ms on
-  // MI355X; at 16 inputs and fewer it measured no better)
-  __device__ static constexpr int swt(int t) { return MB >= 2 ? t & 1 : 0; }
-  // FP32 row swizzle of the 16-byte chunks at 32 inputs: rows t and t+2 of
-  // 128 bytes share a bank half, so every other pair of rows swaps its two
-  // block halves.
-  __device__ static constexpr int swr(int t) { return MB == 2 ? ((t >> 1) & 1) << 2 : 0; }
+  // FP32 word swizzles: at 32 inputs odd rows swap their block halves (rows
+  // of 128 bytes repeat the banks); the low three bits of the input rotate
+  // with the channel pair and the row (t mod 4), which spreads the channel
+  // pairs and rows of one store over the 8-word groups.
+  __device__ static constexpr int swf(int t) { return MB == 2 ? 16 * (t & 1) : 0; }
+  __device__ static constexpr int swq(int t, int cc) { return ((cc >> 1) + 4 * (t & 1) + 2 * ((t >> 1) & 1)) & 7; }
+  static_assert(!F32 || K == 4, "the FP32 swizzles repeat every K rows");
   // byte offset of component (plane, input m, FFT t) within channel cc of
-  // the tile (cc only moves the 16/8-bit chunk); 16/8-bit: t a multiple of 8
-  // (the start of an operand)
+  // the tile; 16/8-bit: t a multiple of 8 (the start of an operand)
   __device__ static constexpr int at(int plane, int m, int t, int cc) {
-    return F32 ? plane * PLANEB + t * ROWB + (((m >> 2) ^ swt(t) ^ swr(t)) << 4) + (m & 3) * 4
+    return F32 ? plane * PLANEB + t * ROWB + 4 * (m ^ swf(t) ^ swq(t, cc))
                : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
   }
   static_assert(F32 || TB / 8 == 4, "4 chunks per row");
@@ -193,18 +188,19 @@ struct Stage {
       w[k] = make_uint4(w[k].x & mk, w[k].y & mk, w[k].z & mk, w[k].w & mk);
     }
     if constexpr (X::F32) {
+      // input 8*mq + k of channel 2q (and 2q + 1) at word (8*mq + k) ^ s:
+      // bits 2..4 of the byte offset o0 hold s mod 8 and nothing else, so
+      // input k's word is o0 ^ 4k
       const int nmq = (M + 7) / 8;
       const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
-      unsigned char *d0 = lds + (q * 2) * X::CHB, *d1 = d0 + X::CHB;
+      const int o0 = (q * 2) * X::CHB + X::at(0, 8 * mq, t, q * 2);
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const uint4 *v = w + 4 * h;
-        if (8 * mq + 4 * h >= X::MP) break;  // rows past the padded block count
-        const int m0 = 8 * mq + 4 * h;
-        *reinterpret_cast<uint4 *>(d0 + X::at(0, m0, t, 0)) = make_uint4(v[0].x, v[1].x, v[2].x, v[3].x);
-        *reinterpret_cast<uint4 *>(d0 + X::at(1, m0, t, 0)) = make_uint4(v[0].y, v[1].y, v[2].y, v[3].y);
-        *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t, 0)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
-        *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t, 0)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
+      for (int k = 0; k < 8; k++) {
+        unsigned char *d = lds + (o0 ^ (k << 2));
+        *reinterpret_cast<uint32_t *>(d) = w[k].x;
+        *reinterpret_cast<uint32_t *>(d + X::PLANEB) = w[k].y;
+        *reinterpret_cast<uint32_t *>(d + X::CHB) = w[k].z;
+        *reinterpret_cast<uint32_t *>(d + X::CHB + X::PLANEB) = w[k].w;
       }
     } else {
       const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
@@ -324,9 +320,12 @@ __device__ __forceinline__ void xmfma_item(const XMParams &p, int group, int spl
 #pragma unroll
         for (int b = 0; b < MB; b++) {
           const int m = b * 16 + (lane & 15);
-          const int t = X::F32 ? ks * X::K + (lane >> 4) : (lane >> 4) * 8;
-          R[b] = *reinterpret_cast<const Op *>(ch + X::at(0, m, t, cc));
-          I[b] = *reinterpret_cast<const Op *>(ch + X::at(1, m, t, cc));
+          // FP32: the swizzles repeat every K = 4 rows, so K-step ks is a
+          // constant offset from the lane's row lane/16
+          const int t = X::F32 ? (lane >> 4) : (lane >> 4) * 8;
+          const unsigned char *op = ch + X::at(0, m, t, cc) + (X::F32 ? ks * X::K * X::ROWB : 0);
+          R[b] = *reinterpret_cast<const Op *>(op);
+          I[b] = *reinterpret_cast<const Op *>(op + X::PLANEB);
         }
         int tile = 0;
 #pragma unroll
@@ -422,8 +421,19 @@ __device__ __forceinline__ long long xcd_swizzle(long long u, long long total) {
   return (u % kXcds) * per + u / kXcds;
 }
 
+// Waves per SIMD the register allocation must leave room for: two
+// workgroups of 8 waves per CU (4 per SIMD, 128 registers per wave) where
+// their LDS tiles fit and the accumulators take at most 96 registers;
+// otherwise what one workgroup needs.
+template <int ET, int MB, int CT, int NWV>
+constexpr int xm_min_waves() {
+  using X = XM<ET, MB, CT, NWV>;
+  return NWV == 8 && X::CPW * X::NTILE * 12 <= 96 && 2 * X::LDSB <= 160 * 1024 ? 4 : (NWV + 3) / 4;
+}
+
 template <int ET, int MB, int CT, int NWV = 4>
-__global__ void __launch_bounds__(64 * NWV) xmfma_kernel(XMParams p) {
+__global__ void __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(xm_min_waves<ET, MB, CT, NWV>())))
+xmfma_kernel(XMParams p) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const int groups = p.nchan / CT;
   const long long u = xcd_swizzle(blockIdx.x, (long long)groups * p.splits);
