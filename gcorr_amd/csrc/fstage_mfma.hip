@@ -129,13 +129,21 @@ struct MG {
   static constexpr int K2 = O / NW;                 // 16-column tiles per wave, last pass
   // LDS word of point i in the buffer pass 0 writes (layout A) and in the
   // one pass 1 writes (layout B): i + a1 (i >> s1) + a2 (i >> s2), chosen so
-  // that the 64 lanes of every LDS access of the passes fall in distinct
-  // banks (64 banks of 4 bytes; the b128 stores in quarter-waves), except
-  // the pass-1 stores at L = 4096 (two lanes to a bank).  Terms of layout A
-  // are multiples of 4, keeping each lane's 16-byte store aligned.  Every
-  // access below is a per-lane base plus an offset that is the same for all
-  // lanes and waves, so the offsets are immediates.
-  static constexpr int A1 = 4, SA1 = 6, A2 = 16, SA2 = L >= 8192 ? 11 : 10;
+  // that each 32-lane half of every 4-byte access of the passes falls in 32
+  // distinct banks (LDS serves a wave's 4-byte accesses as two halves over
+  // 32 banks of 4 bytes) and each 16-lane quarter of the 16-byte pass-0
+  // stores covers 64 distinct banks, except for two lanes to a bank in a
+  // quarter of the last-pass loads at L = 2048 (no layout of this form does
+  // better) and in the pass-1 stores at L = 4096 (the conflict-free layout
+  // B (16, 6, 1, 10) there takes 1.8 KB more per workgroup and measured 1 %
+  // slower in an 8-subband batch on MI355X).  Terms of layout A are
+  // multiples of 4, keeping each lane's 16-byte store aligned.  Every
+  // access below is a per-lane base plus an offset that is the same for
+  // all lanes and waves, so the offsets are immediates.  (At L = 2048,
+  // layout A's second term at bit 9 instead of 10 took the pass-1 loads
+  // from two lanes to a bank to none: F-stage 1.92 -> 1.90 ms at 6
+  // stations x 1024 channels, FP16.)
+  static constexpr int A1 = 4, SA1 = 6, A2 = 16, SA2 = L >= 8192 ? 11 : L == 4096 ? 10 : 9;
   static constexpr int B1 = L == 4096 ? 1 : 16, SB1 = L == 4096 ? 4 : 6;
   static constexpr int B2 = L == 2048 ? 2 : L == 4096 ? 10 : 16, SB2 = L == 4096 ? 7 : 10;
   static constexpr int lay_a(int i) { return i + A1 * (i >> SA1) + A2 * (i >> SA2); }
