@@ -61,39 +61,47 @@ struct XM {
   static constexpr int CPW = CT / NWV;  // channels per wave
   // 16/8-bit: per plane, rows m of TB*EB bytes = TB/8 chunks of 8 FFTs.
   // FP32: per plane, rows t of the MP inputs (64 inputs: rows padded by 16
-  // words), input m of row t of channel cc in word m ^ swf(t) ^ swq(t, cc),
-  // so that one 32-lane half of an MFMA operand read (16 inputs of rows t
-  // and t+1) and one half of a staging store (one word of each lane: 8
-  // channel pairs x 4 eight-input groups) both cover the 32 banks once.
-  // LDS serves 32 lanes x 4 bytes per cycle over 32 banks, and a 16-byte
-  // store as 4 such word passes, so 16-byte stores of aligned chunks hit
-  // every bank at least 4 times: the FP32 staging stores whole words.  (At
-  // 16 stations, FP32, this layout replaced 16-byte stores of 4-input
-  // chunks, which cost 7 extra cycles per word pass on MI355X.)
+  // words).  LDS serves a wave's 4-byte accesses as two 32-lane halves over
+  // 32 banks.  At 17..32 inputs (WS) input m of row t of channel cc sits in
+  // word m ^ swf(t) ^ swq(t, cc), so that one half of an MFMA operand read
+  // (16 inputs of rows t and t+1) and one half of a staging store of single
+  // words (8 channel pairs x 4 eight-input groups) both cover the 32 banks
+  // once; otherwise the staging stores 16-byte chunks of 4 inputs, placed so
+  // that the 16 lanes of a block read 16 distinct banks.  (At 16 stations,
+  // FP32, the word stores took the X-stage from 7.08 to 6.44 ms on MI355X,
+  // LDS bank-conflict cycles from 1.0e9 per launch to none; at 64 inputs,
+  // where no register prefetch hides the staging, they measured 5 % slower
+  // than the 16-byte chunks.)
+  static constexpr bool WS = F32 && MB == 2;
   static constexpr int CHUNKB = 8 * EB;
   static constexpr int ROWB = F32 ? MP * 4 + (MB == 4 ? 64 : 0) : TB * EB;
   static constexpr int PLANEB = F32 ? TB * ROWB : MP * ROWB;
-  // Channel pad: +16 B spreads the 16-bit channels over banks; FP32 and
-  // the 8-bit layout need none (swq, swc).
-  static constexpr int CHB = 2 * PLANEB + (F32 || EB == 1 ? 0 : 16);
+  // Channel pad: +16 B spreads the 16-bit channels and the FP32 chunks over
+  // banks; the word layout (swq) and the 8-bit layout (swc) need none.
+  static constexpr int CHB = 2 * PLANEB + (WS || EB == 1 ? 0 : 16);
   static constexpr int LDSB = CT * CHB;
   // chunk swizzle of the 16/8-bit layout: the 16 inputs of one MFMA block
   // read their chunk t/8 from 16 distinct bank groups (swz), and the channel
   // columns of one staging store spread over the chunks (swc)
   __device__ static constexpr int swz(int m) { return EB == 2 ? (m >> 2) & 3 : ((m >> 2) ^ (m >> 4)) & 3; }
   __device__ static constexpr int swc(int cc) { return EB == 2 ? (cc >> 2) & 3 : (cc >> 3) & 3; }
-  // FP32 word swizzles: at 32 inputs odd rows swap their block halves (rows
-  // of 128 bytes repeat the banks); the low three bits of the input rotate
-  // with the channel pair and the row (t mod 4), which spreads the channel
-  // pairs and rows of one store over the 8-word groups.
-  __device__ static constexpr int swf(int t) { return MB == 2 ? 16 * (t & 1) : 0; }
+  // FP32 word swizzles: odd rows swap their block halves (rows of 128 bytes
+  // repeat the banks); the low three bits of the input rotate with the
+  // channel pair and the row (t mod 4), which spreads the channel pairs and
+  // rows of one store over the 8-word groups.
+  __device__ static constexpr int swf(int t) { return 16 * (t & 1); }
   __device__ static constexpr int swq(int t, int cc) { return ((cc >> 1) + 4 * (t & 1) + 2 * ((t >> 1) & 1)) & 7; }
+  // FP32 chunk swizzle at 64 inputs: odd rows swap neighbouring 4-input
+  // chunks, so that one staging store's lanes cover both 16-byte parities of
+  // the bank groups
+  __device__ static constexpr int swt(int t) { return MB >= 2 ? t & 1 : 0; }
   static_assert(!F32 || K == 4, "the FP32 swizzles repeat every K rows");
   // byte offset of component (plane, input m, FFT t) within channel cc of
   // the tile; 16/8-bit: t a multiple of 8 (the start of an operand)
   __device__ static constexpr int at(int plane, int m, int t, int cc) {
-    return F32 ? plane * PLANEB + t * ROWB + 4 * (m ^ swf(t) ^ swq(t, cc))
-               : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
+    return WS    ? plane * PLANEB + t * ROWB + 4 * (m ^ swf(t) ^ swq(t, cc))
+           : F32 ? plane * PLANEB + t * ROWB + (((m >> 2) ^ swt(t)) << 4) + (m & 3) * 4
+                 : plane * PLANEB + m * ROWB + (((t >> 3) ^ swz(m) ^ swc(cc)) & (TB / 8 - 1)) * CHUNKB;
   }
   static_assert(F32 || TB / 8 == 4, "4 chunks per row");
 };
@@ -187,7 +195,7 @@ struct Stage {
       const uint32_t mk = 0u - ((ok >> k) & 1u);
       w[k] = make_uint4(w[k].x & mk, w[k].y & mk, w[k].z & mk, w[k].w & mk);
     }
-    if constexpr (X::F32) {
+    if constexpr (X::WS) {
       // input 8*mq + k of channel 2q (and 2q + 1) at word (8*mq + k) ^ s:
       // bits 2..4 of the byte offset o0 hold s mod 8 and nothing else, so
       // input k's word is o0 ^ 4k
@@ -201,6 +209,21 @@ struct Stage {
         *reinterpret_cast<uint32_t *>(d + X::PLANEB) = w[k].y;
         *reinterpret_cast<uint32_t *>(d + X::CHB) = w[k].z;
         *reinterpret_cast<uint32_t *>(d + X::CHB + X::PLANEB) = w[k].w;
+      }
+    } else if constexpr (X::F32) {
+      // inputs 8*mq + 4h .. + 3 of channel 2q (and 2q + 1) as one 16-byte chunk
+      const int nmq = (M + 7) / 8;
+      const int q = task % NQ, mq = (task / NQ) % nmq, t = task / (NQ * nmq);
+      unsigned char *d0 = lds + (q * 2) * X::CHB, *d1 = d0 + X::CHB;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint4 *v = w + 4 * h;
+        if (8 * mq + 4 * h >= X::MP) break;  // rows past the padded block count
+        const int m0 = 8 * mq + 4 * h;
+        *reinterpret_cast<uint4 *>(d0 + X::at(0, m0, t, 0)) = make_uint4(v[0].x, v[1].x, v[2].x, v[3].x);
+        *reinterpret_cast<uint4 *>(d0 + X::at(1, m0, t, 0)) = make_uint4(v[0].y, v[1].y, v[2].y, v[3].y);
+        *reinterpret_cast<uint4 *>(d1 + X::at(0, m0, t, 0)) = make_uint4(v[0].z, v[1].z, v[2].z, v[3].z);
+        *reinterpret_cast<uint4 *>(d1 + X::at(1, m0, t, 0)) = make_uint4(v[0].w, v[1].w, v[2].w, v[3].w);
       }
     } else {
       const int q = task % NQ, m = (task / NQ) % M, o = task / (NQ * M);
