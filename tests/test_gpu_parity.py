@@ -206,7 +206,8 @@ def test_process_is_deterministic_and_reusable(gpu):
 # ------------------------------------------------------------------ X-engine KAT
 
 @pytest.mark.parametrize("nant,nchan,nfft", [(3, 256, 4), (5, 1024, 100), (3, 512, 3000),
-                                             (16, 128, 40), (9, 64, 50), (32, 64, 40)])
+                                             (16, 128, 40), (9, 64, 50), (32, 64, 40),
+                                             (10, 64, 33), (12, 32, 17), (13, 64, 50), (15, 64, 24)])
 def test_xengine_integer_kat(gpu, nant, nchan, nfft):
     """gcorr/test/unit.cc:73-159: integer spectra in [-5, 5] make every sum
     exact, so the GPU result equals the exact sum / nfft bit for bit."""
@@ -225,6 +226,24 @@ def test_xengine_integer_kat(gpu, nant, nchan, nfft):
                 expect = (s.real.astype(np.float32) / np.float32(nfft)) \
                     + 1j * (s.imag.astype(np.float32) / np.float32(nfft))
                 np.testing.assert_array_equal(got[r, 2 * l + m], expect.astype(np.complex64))
+
+
+@pytest.mark.parametrize("nant", [9, 10, 12, 13, 15, 16])
+def test_fp32_xtiles_agree_bitwise_17_to_32_inputs(gpu, nant):
+    """At 17..32 inputs the FP32 X-stage stages single words in LDS under an
+    XOR swizzle of input, channel pair and FFT row.  A channel's sums run in
+    the same order whatever the channel tile, so every tile must give the
+    same visibilities bit for bit, and they match the oracle.  70 FFTs in 2
+    splits leave a partial last LDS stage in each split."""
+    w, data, poly, offs = random_case(nant, 256, 70, seed=40 + nant)
+    out = {}
+    for xt in (4, 8, 16):
+        out[xt], rc = run_gpu(w, data, poly, offs, xtile=xt, xsplits=2)
+    ref, rc_ref = run_oracle(w, data, poly, offs)
+    np.testing.assert_array_equal(rc, rc_ref)
+    assert eps(out[16], ref) < EPS
+    for xt in (4, 8):
+        np.testing.assert_array_equal(out[xt], out[16])
 
 
 # ------------------------------------------------------------------ interface
