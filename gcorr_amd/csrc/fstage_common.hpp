@@ -63,6 +63,35 @@ __device__ __forceinline__ float2 twc(float2 a) {
   }
 }
 
+// plus = a + W_N^K b, minus = a - W_N^K b for a compile-time twiddle.
+// Off the multiples of N/4 the twiddle is factored as W = c (1 + i t) with
+// |t| <= 1 (|cos| >= |sin|) or W = s (t + i): b' = (1 + i t) b (or (t + i) b)
+// takes 2 FMAs and a +- c b' 4 more, 6 instructions where a complex product
+// and two complex additions take 8 (the fused multiply-add form of
+// Linzer and Feig, 1993).
+template <int N, int K>
+__device__ __forceinline__ void bfly_tw(float2 a, float2 b, float2 &plus, float2 &minus) {
+  constexpr int k = K % N;
+  if constexpr (k == 0) {
+    plus = cadd(a, b);
+    minus = csub(a, b);
+  } else if constexpr (4 * k == N || 2 * k == N || 4 * k == 3 * N) {
+    const float2 w = twc<N, K>(b);
+    plus = cadd(a, w);
+    minus = csub(a, w);
+  } else {
+    constexpr double ang = 2.0 * kPi * (double)k / (double)N;
+    constexpr double wr = cx_cos(ang), wi = -cx_sin(ang);
+    constexpr bool cosform = (wr < 0 ? -wr : wr) >= (wi < 0 ? -wi : wi);
+    constexpr float c = (float)(cosform ? wr : wi);
+    constexpr float t = (float)(cosform ? wi / wr : wr / wi);
+    const float2 bp = cosform ? make_float2(b.x - t * b.y, b.y + t * b.x)
+                              : make_float2(t * b.x - b.y, t * b.y + b.x);
+    plus = make_float2(a.x + c * bp.x, a.y + c * bp.y);
+    minus = make_float2(a.x - c * bp.x, a.y - c * bp.y);
+  }
+}
+
 __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
   const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
   const float2 t2 = cadd(a1, a3), t3 = mnegi(csub(a1, a3));
@@ -75,6 +104,15 @@ __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 
 template <int K>
 __device__ __forceinline__ float2 tw32(float2 a) {
   return twc<32, K>(a);
+}
+
+// v[q] = e[q] + W32^q o[q], v[q + 16] = e[q] - W32^q o[q] for q = Q .. 15
+template <int Q>
+__device__ __forceinline__ void bfly32(const float2 *e, const float2 *o, float2 *v) {
+  if constexpr (Q < 16) {
+    bfly_tw<32, Q>(e[Q], o[Q], v[Q], v[Q + 16]);
+    bfly32<Q + 1>(e, o, v);
+  }
 }
 
 // In-place forward DFT of R points v[0..R), natural order in and out.
@@ -91,29 +129,40 @@ __device__ __forceinline__ void dft(float2 *v) {
     // k = k1 + 2 k2: DFT4 over k2, twiddle W8^(k1 q2), DFT2 over k1.
     dft4(v[0], v[2], v[4], v[6]);
     dft4(v[1], v[3], v[5], v[7]);
-    const float2 y10 = v[1], y11 = twc<8, 1>(v[3]), y12 = twc<8, 2>(v[5]), y13 = twc<8, 3>(v[7]);
     const float2 y00 = v[0], y01 = v[2], y02 = v[4], y03 = v[6];
-    v[0] = cadd(y00, y10); v[4] = csub(y00, y10);
-    v[1] = cadd(y01, y11); v[5] = csub(y01, y11);
-    v[2] = cadd(y02, y12); v[6] = csub(y02, y12);
-    v[3] = cadd(y03, y13); v[7] = csub(y03, y13);
+    const float2 y10 = v[1], y11 = v[3], y12 = v[5], y13 = v[7];
+    bfly_tw<8, 0>(y00, y10, v[0], v[4]);
+    bfly_tw<8, 1>(y01, y11, v[1], v[5]);
+    bfly_tw<8, 2>(y02, y12, v[2], v[6]);
+    bfly_tw<8, 3>(y03, y13, v[3], v[7]);
   } else if constexpr (R == 16) {
     // k = k1 + 4 k2, q = 4 q1 + q2: DFT4 over k2, twiddle W16^(k1 q2), DFT4 over k1.
     dft4(v[0], v[4], v[8], v[12]);
     dft4(v[1], v[5], v[9], v[13]);
     dft4(v[2], v[6], v[10], v[14]);
     dft4(v[3], v[7], v[11], v[15]);
+    // y[k1][q2] = W16^(k1 q2) v[k1 + 4 q2]; k1 = 2 enters its DFT4 as
+    // y0 +- W16^(2 q2) v, one twiddled butterfly (bfly_tw)
     float2 y[4][4];
 #pragma unroll
-    for (int q2 = 0; q2 < 4; q2++) y[0][q2] = v[4 * q2];
+    for (int q2 = 0; q2 < 4; q2++) {
+      y[0][q2] = v[4 * q2];
+      y[2][q2] = v[2 + 4 * q2];
+    }
     y[1][0] = v[1]; y[1][1] = twc<16, 1>(v[5]); y[1][2] = twc<16, 2>(v[9]);  y[1][3] = twc<16, 3>(v[13]);
-    y[2][0] = v[2]; y[2][1] = twc<16, 2>(v[6]); y[2][2] = twc<16, 4>(v[10]); y[2][3] = twc<16, 6>(v[14]);
     y[3][0] = v[3]; y[3][1] = twc<16, 3>(v[7]); y[3][2] = twc<16, 6>(v[11]); y[3][3] = twc<16, 9>(v[15]);
+    float2 t0[4], t1[4];
+    bfly_tw<16, 0>(y[0][0], y[2][0], t0[0], t1[0]);
+    bfly_tw<16, 2>(y[0][1], y[2][1], t0[1], t1[1]);
+    bfly_tw<16, 4>(y[0][2], y[2][2], t0[2], t1[2]);
+    bfly_tw<16, 6>(y[0][3], y[2][3], t0[3], t1[3]);
 #pragma unroll
     for (int q2 = 0; q2 < 4; q2++) {
-      float2 a0 = y[0][q2], a1 = y[1][q2], a2 = y[2][q2], a3 = y[3][q2];
-      dft4(a0, a1, a2, a3);
-      v[q2] = a0; v[4 + q2] = a1; v[8 + q2] = a2; v[12 + q2] = a3;
+      const float2 t2 = cadd(y[1][q2], y[3][q2]), t3 = mnegi(csub(y[1][q2], y[3][q2]));
+      v[q2] = cadd(t0[q2], t2);
+      v[8 + q2] = csub(t0[q2], t2);
+      v[4 + q2] = cadd(t1[q2], t3);
+      v[12 + q2] = csub(t1[q2], t3);
     }
   } else if constexpr (R == 32) {
     // Even/odd split: X[q] = E[q] + W32^q O[q], X[q+16] = E[q] - W32^q O[q].
@@ -125,15 +174,7 @@ __device__ __forceinline__ void dft(float2 *v) {
     }
     dft<16>(e);
     dft<16>(o);
-    const float2 w[16] = {o[0],         tw32<1>(o[1]),   tw32<2>(o[2]),   tw32<3>(o[3]),
-                          tw32<4>(o[4]), tw32<5>(o[5]),   tw32<6>(o[6]),   tw32<7>(o[7]),
-                          tw32<8>(o[8]), tw32<9>(o[9]),   tw32<10>(o[10]), tw32<11>(o[11]),
-                          tw32<12>(o[12]), tw32<13>(o[13]), tw32<14>(o[14]), tw32<15>(o[15])};
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      v[q] = cadd(e[q], w[q]);
-      v[q + 16] = csub(e[q], w[q]);
-    }
+    bfly32<0>(e, o, v);
   }
 }
 
