@@ -660,11 +660,14 @@ hipError_t launch_xm(const XMParams &p, hipStream_t s) {
 // Channel tiles offered besides the default: half (fewer accumulators,
 // more workgroups) and double (longer rows per load), where the tile holds
 // whole 16-byte columns, every wave has a channel, the accumulators of a
-// wave stay within 192 registers and the LDS tile within 160 KiB.
+// wave stay within 192 registers (96 on 16 waves, whose 128-register cap
+// would otherwise spill: 67-89 registers at 64 inputs, 16 channels) and the
+// LDS tile within 160 KiB.
 template <int ET, int MB, int CT>
 constexpr bool xm_tile_ok() {
   using X = XM<ET, MB, CT>;
-  return CT >= 4 && CT % X::EPL == 0 && (CT / xm_waves<ET, MB, CT>()) * X::NTILE * 12 <= 192 &&
+  constexpr int nwv = xm_waves<ET, MB, CT>();
+  return CT >= 4 && CT % X::EPL == 0 && (CT / nwv) * X::NTILE * 12 <= (nwv >= 16 ? 96 : 192) &&
          X::LDSB <= 160 * 1024;
 }
 
