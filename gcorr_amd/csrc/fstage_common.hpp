@@ -320,20 +320,24 @@ __device__ __forceinline__ float bin_scale(const FParams &p, int srow, int q) {
 
 // Store bin q of output row `row` (= 2*station + pol within FFT row block)
 // in the spectra format OUT, with its scale sc = bin_scale(...) already read.
+// The spectra are written once and read once, by the X-stage after the
+// whole F-stage: streaming (nontemporal) stores.
 template <int OUT, int NCHAN>
 __device__ __forceinline__ void store_bin_sc(const FParams &p, size_t row, int q, float2 y, float sc) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
   if constexpr (OUT == OUT_F32) {
-    p.spectra[row * NCHAN + q] = y;
+    const f32x2 v = {y.x, y.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x2 *>(p.spectra + row * NCHAN + q));
   } else if constexpr (OUT == OUT_POWER) {
     // one row per sampled FFT; the scale kernel sums rows in a fixed order,
     // so the scales do not depend on scheduling
     p.power[row * NCHAN + q] = y.x * y.x + y.y * y.y;
   } else if constexpr (OUT == OUT_FP8) {
-    static_cast<uint16_t *>(p.out)[row * NCHAN + q] = pack_fp8(y.x * sc, y.y * sc);
+    __builtin_nontemporal_store(pack_fp8(y.x * sc, y.y * sc), static_cast<uint16_t *>(p.out) + row * NCHAN + q);
   } else if constexpr (OUT == OUT_F16) {
-    static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_f16(y.x * sc, y.y * sc);
+    __builtin_nontemporal_store(pack_f16(y.x * sc, y.y * sc), static_cast<uint32_t *>(p.out) + row * NCHAN + q);
   } else {
-    static_cast<uint32_t *>(p.out)[row * NCHAN + q] = pack_bf16(y.x * sc, y.y * sc);
+    __builtin_nontemporal_store(pack_bf16(y.x * sc, y.y * sc), static_cast<uint32_t *>(p.out) + row * NCHAN + q);
   }
 }
 

@@ -628,7 +628,8 @@ __global__ void __launch_bounds__(MG<L>::NTHR, 4) fstage_mfma_kernel(FParams p, 
         const float2 y = cmul(fr, make_float2(yr[i], yi[i]));
         if constexpr (OUT == OUT_F16) {
           // the scaled pair rounded to f16 in one conversion (as store_bin_sc)
-          static_cast<uint32_t *>(p.out)[row * NCHAN + qb + 256 * i] = hbits(toh2(y.x * sc[k][i], y.y * sc[k][i]));
+          __builtin_nontemporal_store(hbits(toh2(y.x * sc[k][i], y.y * sc[k][i])),
+                                      static_cast<uint32_t *>(p.out) + row * NCHAN + qb + 256 * i);
         } else {
           store_bin_sc<OUT, NCHAN>(p, row, qb + 256 * i, y, sc[k][i]);
         }

@@ -40,6 +40,7 @@ namespace {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Default channel tile: 16 (FP32) or 32 channels per workgroup over the
 // input blocks.
@@ -183,7 +184,9 @@ struct Stage {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const bool in = k < kmax;
-      w[k] = *reinterpret_cast<const uint4 *>(base + (in ? off0 + k * stride : 0u));
+      // each spectra line is read once: a streaming (nontemporal) load
+      const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(base + (in ? off0 + k * stride : 0u)));
+      w[k] = make_uint4(v.x, v.y, v.z, v.w);
       ok |= (unsigned)in << k;
     }
     return ok;
