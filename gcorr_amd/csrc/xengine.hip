@@ -153,6 +153,12 @@ struct Stage {
   // stage run (at 64 inputs the accumulators take 120 and the prefetch
   // would cost occupancy or spill).
   static constexpr bool PREFETCH = MAXTASK <= X::NTHR && X::CPW * X::NTILE * 12 <= 96;
+  // Each spectra line is read once.  Where the loads run a stage ahead
+  // (PREFETCH), 16- and 32-bit spectra take streaming (nontemporal) loads:
+  // C3 FP32 X-stage 6.40 -> 6.20 ms on MI355X.  Where a stage waits for its
+  // loads (64 inputs) they cost latency (C5 FP32 13.95 -> 15.26 ms), and the
+  // FP8 X-stage lost 4 %, so those keep ordinary loads.
+  static constexpr bool NT = PREFETCH && X::EB >= 2;
 
   // The loads are unconditional (rows past M and FFTs past te read the
   // stage's first word instead, and a stage at or past te, or a task < 0,
@@ -184,9 +190,13 @@ struct Stage {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const bool in = k < kmax;
-      // each spectra line is read once: a streaming (nontemporal) load
-      const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(base + (in ? off0 + k * stride : 0u)));
-      w[k] = make_uint4(v.x, v.y, v.z, v.w);
+      const unsigned char *a = base + (in ? off0 + k * stride : 0u);
+      if constexpr (NT) {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a));
+        w[k] = make_uint4(v.x, v.y, v.z, v.w);
+      } else {
+        w[k] = *reinterpret_cast<const uint4 *>(a);
+      }
       ok |= (unsigned)in << k;
     }
     return ok;
