@@ -322,10 +322,11 @@ __device__ __forceinline__ float bin_scale(const FParams &p, int srow, int q) {
 // in the spectra format OUT, with its scale sc = bin_scale(...) already read.
 // The spectra are written once and read once, by the X-stage after the
 // whole F-stage: streaming (nontemporal) stores of the 4- and 8-byte
-// values (C3 X-stage FP16 3.06 -> 2.78 ms, FP32 F-stage -0.8 % on MI355X,
-// the X-stage no longer meeting the write-back of dirty spectra lines).
-// The 2-byte FP8 values keep ordinary stores, which the caches combine
-// (as streaming stores the C3 FP8 F-stage measured 5 % slower).
+// values, so that the X-stage no longer meets the write-back of dirty
+// spectra lines (16 stations x 4096 channels on MI355X: FP16 X-stage 3.06
+// -> 2.78 ms, FP32 F-stage -0.8 %).  The 2-byte FP8 values keep ordinary
+// stores, which the caches combine (as streaming stores the 16-station FP8
+// F-stage measured 2-5 % slower).
 template <int OUT, int NCHAN>
 __device__ __forceinline__ void store_bin_sc(const FParams &p, size_t row, int q, float2 y, float sc) {
   typedef float f32x2 __attribute__((ext_vector_type(2)));

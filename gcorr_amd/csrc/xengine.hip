@@ -155,9 +155,10 @@ struct Stage {
   static constexpr bool PREFETCH = MAXTASK <= X::NTHR && X::CPW * X::NTILE * 12 <= 96;
   // Each spectra line is read once.  Where the loads run a stage ahead
   // (PREFETCH), 16- and 32-bit spectra take streaming (nontemporal) loads:
-  // C3 FP32 X-stage 6.40 -> 6.20 ms on MI355X.  Where a stage waits for its
-  // loads (64 inputs) they cost latency (C5 FP32 13.95 -> 15.26 ms), and the
-  // FP8 X-stage lost 4 %, so those keep ordinary loads.
+  // 16 stations x 4096 channels, FP32 X-stage 6.40 -> 6.20 ms on MI355X.
+  // Where a stage waits for its loads (64 inputs) they cost latency (32
+  // stations x 32768 channels, FP32, 4-channel tile: 13.9 -> 26.5 ms) and
+  // keep ordinary loads, as does FP8 (the same time either way).
   static constexpr bool NT = PREFETCH && X::EB >= 2;
 
   // The loads are unconditional (rows past M and FFTs past te read the
