@@ -148,18 +148,24 @@ struct Stage {
   // tasks of a stage at M inputs, and at most (M = MP)
   __device__ static int ntask(int M) { return X::F32 ? NQ * ((M + 7) / 8) * X::TB : NQ * M * (X::TB / 8); }
   static constexpr int MAXTASK = X::F32 ? NQ * (X::MP / 8) * X::TB : NQ * X::MP * (X::TB / 8);
-  // One task per thread and at most 96 accumulator registers per wave: the
-  // next stage's 8 loads (32 registers) can be held while the MFMAs of this
-  // stage run (at 64 inputs the accumulators take 120 and the prefetch
-  // would cost occupancy or spill).
-  static constexpr bool PREFETCH = MAXTASK <= X::NTHR && X::CPW * X::NTILE * 12 <= 96;
-  // Each spectra line is read once.  Where the loads run a stage ahead
-  // (PREFETCH), 16- and 32-bit spectra take streaming (nontemporal) loads:
-  // 16 stations x 4096 channels, FP32 X-stage 6.40 -> 6.20 ms on MI355X.
-  // Where a stage waits for its loads (64 inputs) they cost latency (32
-  // stations x 32768 channels, FP32, 4-channel tile: 13.9 -> 26.5 ms) and
-  // keep ordinary loads, as does FP8 (the same time either way).
-  static constexpr bool NT = PREFETCH && X::EB >= 2;
+  // One task per thread, and registers for the next stage's 8 loads (32)
+  // while the MFMAs of this stage run: at most 96 accumulator registers per
+  // wave, or an 8-wave tile at 64 inputs, which already holds more than
+  // 128 registers (one workgroup per CU, 2 waves per SIMD) and keeps that
+  // occupancy at about 200.  The 4-wave tile at 64 inputs (236 registers,
+  // two workgroups per CU) would drop to one.
+  static constexpr bool PREFETCH =
+      MAXTASK <= X::NTHR && (X::CPW * X::NTILE * 12 <= 96 || (MB == 4 && NWV == 8));
+  // Where the loads run a stage ahead (PREFETCH) and a tile's rows are whole
+  // 128-byte lines, each spectra line is read once, by one workgroup: 16-
+  // and 32-bit spectra then take streaming (nontemporal) loads (16 stations
+  // x 4096 channels, FP32 X-stage 6.40 -> 6.20 ms on MI355X).  Narrower
+  // tiles share lines with their neighbours through L2, which streaming
+  // loads give up (32 stations x 32768 channels, FP16, 8 channels of 32
+  // bytes: X-stage 6.3 -> 8.8 ms); where a stage waits for its loads they
+  // cost latency (FP32, 4-channel tile: 13.9 -> 26.5 ms); FP8 measured the
+  // same either way.  Those keep ordinary loads.
+  static constexpr bool NT = PREFETCH && X::EB >= 2 && X::CT * X::CB >= 128;
 
   // The loads are unconditional (rows past M and FFTs past te read the
   // stage's first word instead, and a stage at or past te, or a task < 0,
