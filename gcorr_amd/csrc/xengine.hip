@@ -71,8 +71,8 @@ struct XM {
   // that the 16 lanes of a block read 16 distinct banks.  (At 16 stations,
   // FP32, the word stores took the X-stage from 7.08 to 6.44 ms on MI355X,
   // LDS bank-conflict cycles from 1.0e9 per launch to none; at 64 inputs,
-  // where no register prefetch hides the staging, they measured 5 % slower
-  // than the 16-byte chunks.)
+  // with the 4-channel tile, which loads and stores each stage in turn,
+  // they measured 5 % slower than the 16-byte chunks.)
   static constexpr bool WS = F32 && MB == 2;
   static constexpr int CHUNKB = 8 * EB;
   static constexpr int ROWB = F32 ? MP * 4 + (MB == 4 ? 64 : 0) : TB * EB;
@@ -137,8 +137,9 @@ struct Operand {
 // stored transposed into LDS as tasks strided over the workgroup's threads;
 // in a task the channel column is the fastest index, so neighbouring lanes
 // load neighbouring 16-byte columns of one spectra row.
-//   FP32: task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads, then per
-//     channel, plane and 4 inputs one 16-byte store;
+//   FP32: task = (column q, inputs 8*mq..8*mq+7, FFT t): 8 loads, then at
+//     17..32 inputs one word store per input, channel and plane, otherwise
+//     one 16-byte store per channel, plane and 4 inputs;
 //   16/8-bit: task = (column q, input m, FFTs 8*o..8*o+7): 8 loads, then per
 //     channel and plane one store of the 8 FFTs' components.
 template <int ET, int MB, int CT, int NWV = 4>
