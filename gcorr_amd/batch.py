@@ -35,7 +35,7 @@ class Batch:
         c.chunks = int(chunks)
         c.overlap = int(bool(overlap))
         c.graph = int(bool(graph))
-        c.fstage_mfma = int(bool(fstage_mfma))
+        c.fstage_mfma = 2 if fstage_mfma == 2 else int(bool(fstage_mfma))
         self.nant = int(nant)
         self.nrows = num_rows(nant, autos)
         h = ctypes.c_void_p()

@@ -199,8 +199,8 @@ gxc_status build_tables(gxc_batch *b) {
   const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
   // with fstage_mfma, the subbands of 2048 points and more get launches of
   // their own: on the matrix cores where those cover them, else on the
-  // one-pol kernel
-  auto on_mfma = [&](const Sub &u) { return b->cfg.fstage_mfma && u.d.L >= 2048; };
+  // one-pol kernel (fstage_mfma 1), or all on the one-pol kernel (2)
+  auto on_mfma = [&](const Sub &u) { return b->cfg.fstage_mfma != 0 && u.d.L >= 2048; };
   b->nmulti = 0;
   for (const Sub &u : b->sub) b->nmulti += on_mfma(u) ? 0 : 1;
   b->mseg.assign(C, {});
@@ -350,7 +350,8 @@ gxc_status enqueue_device_work(gxc_batch *b) {
       GXB_HIP(b, gxc::launch_fstage_multi(b->d_fseg + (size_t)c * b->nmulti, b->nmulti, b->ftotal[c], b->out_mode,
                                           b->fblock, fs));
     for (const gxc::FSeg &f : b->mseg[c])
-      GXB_HIP(b, gxc::fstage_mfma_supported(f.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32)
+      GXB_HIP(b, b->cfg.fstage_mfma == 1 &&
+                         gxc::fstage_mfma_supported(f.L, b->cfg.is_complex != 0, b->out_mode == gxc::OUT_F32)
                      ? gxc::launch_fstage_mfma(f.p, f.L, fs)
                      : gxc::launch_fstage_big(f.p, f.L, b->cfg.is_complex != 0, fs));
     mark(b, 1, fs);
@@ -497,6 +498,8 @@ gxc_status gxc_batch_create(const gxc_batch_config *cfg, gxc_batch **out) {
   if (cfg->nsub > 1024) return set_err(nullptr, GXC_ERR_UNSUPPORTED, "at most 1024 subbands");
   if (cfg->nbit != 2 || cfg->is_complex)
     return set_err(nullptr, GXC_ERR_UNSUPPORTED, "batches take real 2-bit input");
+  if (cfg->fstage_mfma < 0 || cfg->fstage_mfma > 2)
+    return set_err(nullptr, GXC_ERR_INVALID_ARG, "fstage_mfma is 0, 1 or 2");
   std::vector<Sub> subs(cfg->nsub);
   int maxL = 0, minfft = 1 << 30;
   const int out_mode = gxc::host::out_mode_of(cfg->precision < 0 || cfg->precision > 3 ? 0 : cfg->precision);
@@ -852,7 +855,7 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
   int best_fb = b->fblock, best_units = b->xunits, best_mfma = b->cfg.fstage_mfma;
   bool any_mfma = false;  // a subband that fstage_mfma would give its own launch
   for (const Sub &u : b->sub) any_mfma = any_mfma || u.d.L >= 2048;
-  for (int mf = 0; mf <= (any_mfma ? 1 : 0); mf++)
+  for (int mf = 0; mf <= (any_mfma ? 2 : 0); mf++)
     for (int fb : {256, 512}) {
       if (st != GXC_OK) continue;
       for (int units : {1024, 2048, 4096, 8192}) {
@@ -886,7 +889,7 @@ gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out) {
   b->graph_exec = nullptr;
   b->profiling = prof;
   if (st != GXC_OK) return st;
-  t.fblock = b->cfg.fstage_mfma ? gxc::kFBlockMfma : b->fblock;
+  t.fblock = b->cfg.fstage_mfma == 1 ? gxc::kFBlockMfma : b->cfg.fstage_mfma == 2 ? GXC_FBLOCK_ONEPOL : b->fblock;
   t.xtile = gxc::xm_channels_per_group(b->M, b->out_mode);
   t.xsplits = b->xunits;  // for a batch: the X units per launch the splits aim at
   t.graph = b->cfg.graph;

@@ -102,6 +102,10 @@ typedef struct {
  * 2048..8192 for the reduced precisions, 2048..4096 for FP32 (operands as
  * f16 pairs plus remainders, within the FP32 bound). */
 #define GXC_FBLOCK_MFMA (-1)
+/* In a batch's tuning result: the subbands of 2048 points and more have
+ * launches of their own on the one-pol F-stage (gxc_batch_config.fstage_mfma
+ * 2).  No counterpart in the reference (its subbands are separate runs). */
+#define GXC_FBLOCK_ONEPOL (-2)
 
 typedef struct gxc_handle gxc_handle;
 
@@ -431,7 +435,8 @@ typedef struct {
   int fstage_mfma;              /* 1: subbands of 2048 points and more get a
                                    launch each, on the matrix-core F-stage
                                    (GXC_FBLOCK_MFMA) where it covers them,
-                                   else on the one-pol F-stage; the others
+                                   else on the one-pol F-stage; 2: a launch
+                                   each on the one-pol F-stage; the others
                                    stay in the multi-subband launch.
                                    gxc_batch_autotune chooses. */
   int reserved[3];
@@ -477,9 +482,10 @@ gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n
  * without launches of their own for the subbands of 2048 points and more
  * (gxc_batch_config.fstage_mfma), then with and without graph replay, and
  * keeps the fastest.  In the result, fblock is the multi-subband workgroup
- * size, or GXC_FBLOCK_MFMA when the own launches were chosen (the choice is
- * kept in the batch); xsplits holds the chosen X unit count and
- * fstage_ms/xstage_ms are 0. */
+ * size, or GXC_FBLOCK_MFMA / GXC_FBLOCK_ONEPOL when the own launches (on
+ * the matrix cores where they cover the length / on the one-pol kernel)
+ * were chosen (the choice is kept in the batch); xsplits holds the chosen
+ * X unit count and fstage_ms/xstage_ms are 0. */
 gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out);
 
 /* Timing over all subbands (launch counts are per-stage launches, one per

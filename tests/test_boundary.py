@@ -160,7 +160,10 @@ def test_batch_create_checks_config_before_device():
     assert create([(64, 100, 1.65e9, 16e6)]) == 2              # FFT length 128 < 256
     assert create([(8192, 10, 1.65e9, 16e6)]) == 2             # FFT length 16384 > 8192
     assert create([(100, 10, 1.65e9, 16e6)]) == 1              # not a power of two
+    assert create(ok, fstage_mfma=3) == 1                      # own launches: 0, 1 or 2
+    assert create(ok, fstage_mfma=-1) == 1
     assert create(ok) == 3                                     # valid, but no GPU here
+    assert create(ok, fstage_mfma=2) == 3
     msg = L.gxc_batch_last_error(None).decode()
     assert "device" in msg
 

@@ -79,16 +79,18 @@ def test_batch_fp16_vs_fp64_oracle(gpu):
         assert e < 1e-3, s
 
 
-@pytest.mark.parametrize("precision,bound,mode", [("fp32", EPS, "f32"), ("fp16", 1e-3, "f64")])
-def test_batch_matrix_core_fstage(gpu, precision, bound, mode):
-    """fstage_mfma: the 2048..8192-point subbands get a launch each, on the
+@pytest.mark.parametrize("precision,bound,mode,own", [("fp32", EPS, "f32", 1), ("fp16", 1e-3, "f64", 1),
+                                                      ("fp32", EPS, "f32", 2), ("fp16", 1e-3, "f64", 2)])
+def test_batch_matrix_core_fstage(gpu, precision, bound, mode, own):
+    """fstage_mfma 1: the 2048..8192-point subbands get a launch each, on the
     matrix-core F-stage (2048..4096 points for FP32) or else the one-pol
-    F-stage; the others stay in the multi launch."""
+    F-stage; 2: a launch each on the one-pol F-stage; the others stay in
+    the multi launch."""
     ws = subbands(div=125)
     gen = [synth.generate(w, device="cpu") for w in ws]
     datas = [[b.numpy() for b in g[0]] for g in gen]
     delays = [(g[1], g[2]) for g in gen]
-    got = run_batch(ws, datas, delays, precision=precision, fstage_mfma=True, chunks=2)
+    got = run_batch(ws, datas, delays, precision=precision, fstage_mfma=own, chunks=2)
     for s, w in enumerate(ws):
         v, c = got[s]
         ref, rc = oracle(w, datas[s], *delays[s], mode=mode)
