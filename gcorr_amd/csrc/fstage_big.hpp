@@ -203,10 +203,13 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
 #pragma unroll
     for (int k = 0; k < 32; k++) rawin[k] = w[(size_t)k * T];
   } else {
-    // T is even, so every k shares the nibble parity of s0
+    // T is even, so every k shares the nibble parity of s0.  Each byte comes
+    // in its aligned 32-bit word (never past the page of a valid byte): the
+    // field extract below then needs no byte mask, one VALU less per sample.
     gbyte *bsrc = src + (s0 >> 1);
+    gint *wsrc = (gint *)((uintptr_t)bsrc & ~(uintptr_t)3);
 #pragma unroll
-    for (int k = 0; k < 32; k++) rawin[k] = bsrc[(size_t)k * (T / 2)];
+    for (int k = 0; k < 32; k++) rawin[k] = wsrc[(size_t)k * (T / 8)];
   }
   fill_tables<L, CPLX>(p, frac, a, b, stab, j, T);
   // 2-bit levels by code (fxkernel.cpp:15-28): an LDS read replaces the
@@ -235,11 +238,12 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
       v[k] = cmul(cmul(rsub, step[j / S2 + k * (T / S2)]), make_float2(re, im));
     }
   } else {
-    // sample s is nibble s&1 of its byte; pol A in the nibble's low 2 bits
-    const int sh = (s0 & 1) * 4 + 2 * pol;
+    // sample s is nibble s&1 of its byte, the byte (address & 3) of its word;
+    // pol A in the nibble's low 2 bits
+    const int sh = 8 * (int)((uintptr_t)(src + (s0 >> 1)) & 3) + (s0 & 1) * 4 + 2 * pol;
 #pragma unroll
     for (int k = 0; k < 32; k++) {
-      const int code = (rawin[k] >> sh) & 3;
+      const int code = (int)__builtin_amdgcn_ubfe((unsigned)rawin[k], (unsigned)sh, 2u);
       const float x = lut[code];
       const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
       v[k] = make_float2(x * r.x, x * r.y);
@@ -259,7 +263,11 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
 #pragma unroll
     for (int k = 0; k < KEEP; k++) {
       const int q = jj + k * (L / R);
-      const float2 fr = cmul(fsub[q % S], fstep[q / S]);  // fracsamprotator, fxkernel.cpp:614-625
+      // q % S = j % S and q / S = j / S + a compile-time step when S divides
+      // T and L/R, which keeps the table offsets out of the VALU
+      const float2 fr = (T % S == 0 && (L / R) % S == 0)
+                            ? cmul(fsub[j % S], fstep[j / S + (bf * T + k * (L / R)) / S])
+                            : cmul(fsub[q % S], fstep[q / S]);  // fracsamprotator, fxkernel.cpp:614-625
       store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, cmul(fr, v[bf * R + k]));
     }
   }
