@@ -159,10 +159,13 @@ struct BLds {
   static constexpr int N = PLANE2 + TabGeom<L, CPLX>::NTAB + 2;  // + 4-float 2-bit LUT
 };
 
-// Unit u = 2 * (station + nant * FFT) + pol of one launch, on a workgroup
-// of exactly BG::T threads (every thread calls it with the same u: it
-// contains __syncthreads).  smem holds BLds::N float2 slots.
-template <int L, bool CPLX, int OUT>
+// Unit u = 2 * (station + nant * FFT) + pol of one launch (BOTH false), or
+// u = station + nant * FFT with the two polarisations in turn (BOTH true:
+// the delay split, the rotator tables and the input words, which hold both
+// polarisations' samples, then serve two FFTs), on a workgroup of exactly
+// BG::T threads (every thread calls it with the same u: it contains
+// __syncthreads).  smem holds BLds::N float2 slots.
+template <int L, bool CPLX, int OUT, bool BOTH = false>
 __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 *smem) {
   using G = BG<L, CPLX>;
   using TG = typename G::TG;
@@ -172,22 +175,23 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
   float2 *stab = smem + BLds<L, CPLX>::PLANE2;
 
   const int j = threadIdx.x;
-  const int pol = u & 1;
-  const int item = u >> 1;
+  const int pol0 = BOTH ? 0 : u & 1;
+  const int item = BOTH ? u : u >> 1;
   const int tl = item / p.nant;
   const int st = item % p.nant;
   const int t = p.t0 + tl * p.tstride;
-  const size_t row = ((size_t)tl * p.nant + st) * 2 + pol;  // output row
+  const size_t row0 = ((size_t)tl * p.nant + st) * 2;  // output row of pol 0
 
   int offset = 0;
   double frac = 0.0, a = 0.0, b = 0.0;
   // The same for every thread of the workgroup: the branches below that
   // depend on it are uniform, so every __syncthreads is reached by all.
   const bool valid = split_offset<L>(p, st, t, offset, frac, a, b);
-  if (j == 0 && pol == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
+  if (j == 0 && pol0 == 0 && OUT != OUT_POWER) p.valid[(size_t)st * p.numffts + t] = valid ? 1 : 0;
   if (!valid) {
-    for (int q = j; q < TG::NCHAN; q += T)
-      store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, make_float2(0.f, 0.f));
+    for (int pol = pol0; pol < (BOTH ? 2 : pol0 + 1); pol++)
+      for (int q = j; q < TG::NCHAN; q += T)
+        store_bin<OUT, TG::NCHAN>(p, row0 + pol, 2 * st + pol, q, make_float2(0.f, 0.f));
     return;
   }
   // This thread's pass-0 input words, requested before the table build so
@@ -198,78 +202,95 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
   gbyte *src = (gbyte *)p.station[st];
   const int s0 = offset + j;
   int rawin[32];
-  if constexpr (CPLX) {
-    gint *w = reinterpret_cast<gint *>(src + 4 * (size_t)s0);
+  // T is even, so every k shares the nibble parity of s0.  Each byte comes
+  // in its aligned 32-bit word (never past the page of a valid byte): the
+  // field extract below then needs no byte mask, one VALU less per sample.
+  gint *wsrc = CPLX ? reinterpret_cast<gint *>(src + 4 * (size_t)s0)
+                    : (gint *)((uintptr_t)(src + (s0 >> 1)) & ~(uintptr_t)3);
+  constexpr int WSTRIDE = CPLX ? T : T / 8;  // words between points k and k+1
+  auto load_words = [&]() {
 #pragma unroll
-    for (int k = 0; k < 32; k++) rawin[k] = w[(size_t)k * T];
-  } else {
-    // T is even, so every k shares the nibble parity of s0.  Each byte comes
-    // in its aligned 32-bit word (never past the page of a valid byte): the
-    // field extract below then needs no byte mask, one VALU less per sample.
-    gbyte *bsrc = src + (s0 >> 1);
-    gint *wsrc = (gint *)((uintptr_t)bsrc & ~(uintptr_t)3);
-#pragma unroll
-    for (int k = 0; k < 32; k++) rawin[k] = wsrc[(size_t)k * (T / 8)];
-  }
+    for (int k = 0; k < 32; k++) rawin[k] = wsrc[(size_t)k * WSTRIDE];
+  };
+  load_words();
   fill_tables<L, CPLX>(p, frac, a, b, stab, j, T);
   // 2-bit levels by code (fxkernel.cpp:15-28): an LDS read replaces the
   // compare/select chain that would otherwise cost ~6 VALU per sample
   float *lut = reinterpret_cast<float *>(stab + TabGeom<L, CPLX>::NTAB);
   if (!CPLX && j < 4) lut[j] = j == 0 ? -kHiMag : j == 1 ? -1.0f : j == 2 ? 1.0f : kHiMag;
   __syncthreads();
-  const float2 *sub = stab;
-  const float2 *step = sub + S2;
-  const float2 *fsub = step + NSTEPS;
-  const float2 *fstep = fsub + S;
-  const float2 *__restrict__ tw = p.twiddle;
 
-  // Pass-0 point k of thread j is window sample n = j + k*T: its rotator is
-  // sub[n % S2] * step[n / S2] (fxkernel.cpp:553-558) = sub[j % S2] *
-  // step[j / S2 + k*T/S2].
-  const float2 rsub = sub[j % S2];
+  auto one_pol = [&](const int pol) {
+    // An opaque copy of the thread index (and an opaque zero offset) per
+    // polarisation keeps the compiler from hoisting the per-thread LDS and
+    // global addresses, the rotator, twiddle and table reads and their
+    // products out of the loop, where they would hold ~100 registers over
+    // both FFTs.
+    int z = 0, j = threadIdx.x;
+    if constexpr (BOTH) asm volatile("" : "+s"(z), "+v"(j));
+    const float2 *sub = stab + z;
+    const float2 *step = sub + S2;
+    const float2 *fsub = step + NSTEPS;
+    const float2 *fstep = fsub + S;
+    const float2 *__restrict__ tw = p.twiddle + z;
 
-  float2 v[32];
-  if constexpr (CPLX) {
-    const int sh = 16 * pol;
+    // Pass-0 point k of thread j is window sample n = j + k*T: its rotator
+    // is sub[n % S2] * step[n / S2] (fxkernel.cpp:553-558) = sub[j % S2] *
+    // step[j / S2 + k*T/S2].
+    const float2 rsub = sub[j % S2];
+    const size_t row = row0 + pol;
+    float2 v[32];
+    if constexpr (CPLX) {
+      const int sh = 16 * pol;
 #pragma unroll
-    for (int k = 0; k < 32; k++) {
-      const int wv = rawin[k];
-      const float re = (float)(int8_t)((wv >> sh) & 0xff), im = (float)(int8_t)((wv >> (sh + 8)) & 0xff);
-      v[k] = cmul(cmul(rsub, step[j / S2 + k * (T / S2)]), make_float2(re, im));
+      for (int k = 0; k < 32; k++) {
+        const int wv = rawin[k];
+        const float re = (float)(int8_t)((wv >> sh) & 0xff), im = (float)(int8_t)((wv >> (sh + 8)) & 0xff);
+        v[k] = cmul(cmul(rsub, step[j / S2 + k * (T / S2)]), make_float2(re, im));
+      }
+    } else {
+      // sample s is nibble s&1 of its byte, the byte (address & 3) of its
+      // word; pol A in the nibble's low 2 bits
+      const int sh = 8 * (int)((uintptr_t)(src + (s0 >> 1)) & 3) + (s0 & 1) * 4 + 2 * pol;
+#pragma unroll
+      for (int k = 0; k < 32; k++) {
+        const int code = (int)__builtin_amdgcn_ubfe((unsigned)rawin[k], (unsigned)sh, 2u);
+        const float x = lut[code];
+        const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
+        v[k] = make_float2(x * r.x, x * r.y);
+      }
     }
-  } else {
-    // sample s is nibble s&1 of its byte, the byte (address & 3) of its word;
-    // pol A in the nibble's low 2 bits
-    const int sh = 8 * (int)((uintptr_t)(src + (s0 >> 1)) & 3) + (s0 & 1) * 4 + 2 * pol;
 #pragma unroll
-    for (int k = 0; k < 32; k++) {
-      const int code = (int)__builtin_amdgcn_ubfe((unsigned)rawin[k], (unsigned)sh, 2u);
-      const float x = lut[code];
-      const float2 r = cmul(rsub, step[j / S2 + k * (T / S2)]);
-      v[k] = make_float2(x * r.x, x * r.y);
-    }
-  }
-#pragma unroll
-  for (int bf = 0; bf < 32 / G::radix(0); bf++) dft<G::radix(0)>(&v[bf * G::radix(0)]);
-  later_passes<L, CPLX, 1>(v, splane, j, tw);
+    for (int bf = 0; bf < 32 / G::radix(0); bf++) dft<G::radix(0)>(&v[bf * G::radix(0)]);
+    later_passes<L, CPLX, 1>(v, splane, j, tw);
 
-  // last pass outputs: bins q = jj + k*(L/R), keep q < nchan
-  constexpr int PL = G::NPASS - 1;
-  constexpr int R = G::radix(PL), NB = 32 / R;
-  constexpr int KEEP = CPLX ? R : R / 2;
+    // last pass outputs: bins q = jj + k*(L/R), keep q < nchan
+    constexpr int PL = G::NPASS - 1;
+    constexpr int R = G::radix(PL), NB = 32 / R;
+    constexpr int KEEP = CPLX ? R : R / 2;
 #pragma unroll
-  for (int bf = 0; bf < NB; bf++) {
-    const int jj = j + bf * T;
+    for (int bf = 0; bf < NB; bf++) {
+      const int jj = j + bf * T;
 #pragma unroll
-    for (int k = 0; k < KEEP; k++) {
-      const int q = jj + k * (L / R);
-      // q % S = j % S and q / S = j / S + a compile-time step when S divides
-      // T and L/R, which keeps the table offsets out of the VALU
-      const float2 fr = (T % S == 0 && (L / R) % S == 0)
-                            ? cmul(fsub[j % S], fstep[j / S + (bf * T + k * (L / R)) / S])
-                            : cmul(fsub[q % S], fstep[q / S]);  // fracsamprotator, fxkernel.cpp:614-625
-      store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, cmul(fr, v[bf * R + k]));
+      for (int k = 0; k < KEEP; k++) {
+        const int q = jj + k * (L / R);
+        // q % S = j % S and q / S = j / S + a compile-time step when S
+        // divides T and L/R, which keeps the table offsets out of the VALU
+        const float2 fr = (T % S == 0 && (L / R) % S == 0)
+                              ? cmul(fsub[j % S], fstep[j / S + (bf * T + k * (L / R)) / S])
+                              : cmul(fsub[q % S], fstep[q / S]);  // fracsamprotator, fxkernel.cpp:614-625
+        store_bin<OUT, TG::NCHAN>(p, row, 2 * st + pol, q, cmul(fr, v[bf * R + k]));
+      }
     }
+  };
+  one_pol(pol0);
+  if constexpr (BOTH) {
+    // the second polarisation reads its words again (from L2): keeping
+    // them in registers over the first FFT would cost occupancy.  The last
+    // pass of the first FFT read the plane before a barrier, so this FFT's
+    // first plane store is safe.
+    load_words();
+    one_pol(1);
   }
 }
 
