@@ -20,8 +20,8 @@ class Batch:
 
     fstage_mfma: 0 (False) keeps every subband in the multi-subband F launch;
     1 (True) gives the subbands of 2048 points and more a launch each on the
-    matrix-core F-stage where it covers them, else the one-pol F-stage; 2
-    gives them a launch each on the one-pol F-stage.  autotune() chooses.
+    matrix-core F-stage where it covers them, else the register F-stage; 2
+    gives them a launch each on the register F-stage.  autotune() chooses.
     """
 
     def __init__(self, nant, subbands, *, precision="fp32", autos=False, exact_phase=False,

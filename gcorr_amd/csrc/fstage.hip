@@ -24,7 +24,7 @@ int fstage_blocks(int L, bool cplx, int *out) {
     const int b = fs::fblock_for(L, cplx, mult);
     if (b) out[n++] = b;
   }
-  // the one-polarisation-per-workgroup kernel of the long FFTs, L/32 threads
+  // the register kernel of the long FFTs (fstage_big), L/32 threads
   if (L >= 2048) out[n++] = L / 32;
   return n;
 }

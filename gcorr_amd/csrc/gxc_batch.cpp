@@ -199,7 +199,7 @@ gxc_status build_tables(gxc_batch *b) {
   const int ct = gxc::xm_channels_per_group(b->M, b->out_mode);
   // with fstage_mfma, the subbands of 2048 points and more get launches of
   // their own: on the matrix cores where those cover them, else on the
-  // one-pol kernel (fstage_mfma 1), or all on the one-pol kernel (2)
+  // register kernel (fstage_mfma 1), or all on the register kernel (2)
   auto on_mfma = [&](const Sub &u) { return b->cfg.fstage_mfma != 0 && u.d.L >= 2048; };
   b->nmulti = 0;
   for (const Sub &u : b->sub) b->nmulti += on_mfma(u) ? 0 : 1;

@@ -103,8 +103,10 @@ typedef struct {
  * f16 pairs plus remainders, within the FP32 bound). */
 #define GXC_FBLOCK_MFMA (-1)
 /* In a batch's tuning result: the subbands of 2048 points and more have
- * launches of their own on the one-pol F-stage (gxc_batch_config.fstage_mfma
- * 2).  No counterpart in the reference (its subbands are separate runs). */
+ * launches of their own on the register F-stage (gxc_batch_config.fstage_mfma
+ * 2; the name is from when its workgroups took one polarisation each, as
+ * they still do above 8192 points).  No counterpart in the reference (its
+ * subbands are separate runs). */
 #define GXC_FBLOCK_ONEPOL (-2)
 
 typedef struct gxc_handle gxc_handle;
@@ -435,8 +437,8 @@ typedef struct {
   int fstage_mfma;              /* 1: subbands of 2048 points and more get a
                                    launch each, on the matrix-core F-stage
                                    (GXC_FBLOCK_MFMA) where it covers them,
-                                   else on the one-pol F-stage; 2: a launch
-                                   each on the one-pol F-stage; the others
+                                   else on the register F-stage; 2: a launch
+                                   each on the register F-stage; the others
                                    stay in the multi-subband launch.
                                    gxc_batch_autotune chooses. */
   int reserved[3];
@@ -483,7 +485,7 @@ gxc_status gxc_batch_get_row_counts(gxc_batch *b, int sub, int *counts, size_t n
  * (gxc_batch_config.fstage_mfma), then with and without graph replay, and
  * keeps the fastest.  In the result, fblock is the multi-subband workgroup
  * size, or GXC_FBLOCK_MFMA / GXC_FBLOCK_ONEPOL when the own launches (on
- * the matrix cores where they cover the length / on the one-pol kernel)
+ * the matrix cores where they cover the length / on the register kernel)
  * were chosen (the choice is kept in the batch); xsplits holds the chosen
  * X unit count and fstage_ms/xstage_ms are 0. */
 gxc_status gxc_batch_autotune(gxc_batch *b, gxc_tuning *out);

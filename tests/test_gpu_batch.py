@@ -83,8 +83,8 @@ def test_batch_fp16_vs_fp64_oracle(gpu):
                                                       ("fp32", EPS, "f32", 2), ("fp16", 1e-3, "f64", 2)])
 def test_batch_matrix_core_fstage(gpu, precision, bound, mode, own):
     """fstage_mfma 1: the 2048..8192-point subbands get a launch each, on the
-    matrix-core F-stage (2048..4096 points for FP32) or else the one-pol
-    F-stage; 2: a launch each on the one-pol F-stage; the others stay in
+    matrix-core F-stage (2048..4096 points for FP32) or else the register
+    F-stage; 2: a launch each on the register F-stage; the others stay in
     the multi launch."""
     ws = subbands(div=125)
     gen = [synth.generate(w, device="cpu") for w in ws]

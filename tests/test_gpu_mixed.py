@@ -86,7 +86,7 @@ def test_mixed_process_vs_fp64_oracle(gpu, precision):
 def test_c3_shape_every_fstage_vs_fp64_oracle(gpu, precision, fblock):
     """The 16-station, 4096-channel tones workload's shape (600 of its 31 250
     FFTs) through each F-stage the autotuner chooses between at 8192 points:
-    the matrix-core kernel (-1), the one-polarisation kernel (256 threads)
+    the matrix-core kernel (-1), the register kernel (256 threads)
     and the two-polarisation item kernel (default), against the FP64 oracle."""
     w = synth.WORKLOADS["c3"].scaled(600)
     v, ref = _mixed_vs_oracle(w, precision, fblock=fblock)

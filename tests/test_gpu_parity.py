@@ -458,8 +458,9 @@ def test_launch_shape_variants_match_oracle(gpu, fblock_mult, xtile_mult):
 @pytest.mark.parametrize("nchan,is_complex", [(1024, False), (2048, False), (4096, False),
                                               (2048, True), (4096, True), (8192, True)])
 def test_long_fft_kernel_at_short_lengths(gpu, nchan, is_complex):
-    """At FFT lengths 2048..8192 the one-polarisation-per-workgroup kernel
-    (fblock = L/32) is an alternative the autotuner can choose."""
+    """At FFT lengths 2048..8192 the register kernel (fblock = L/32; both
+    polarisations of a station and FFT per workgroup) is an alternative the
+    autotuner can choose."""
     w, data, poly, offs = random_case(3, nchan, 40, seed=15 + nchan, is_complex=is_complex,
                                       bw=128e6 if is_complex else 64e6)
     L = w.fftchannels
