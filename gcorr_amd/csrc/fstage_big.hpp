@@ -196,8 +196,8 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
   }
   // This thread's pass-0 input words, requested before the table build so
   // that their latency overlaps it: complex 8-bit [Are, Aim, Bre, Bim] as
-  // one 32-bit word per sample; 2-bit as the byte holding the sample (byte
-  // layout [A_t, B_t, A_t+1, B_t+1], fxkernel.cpp:443-476).  Pass-0 point k
+  // one 32-bit word per sample; 2-bit as the aligned word that holds the
+  // sample's byte (byte layout [A_t, B_t, A_t+1, B_t+1], fxkernel.cpp:443-476).  Pass-0 point k
   // of thread j is window sample n = j + k*T, packed sample offset + n.
   gbyte *src = (gbyte *)p.station[st];
   const int s0 = offset + j;
@@ -221,11 +221,11 @@ __device__ __forceinline__ void fstage_big_item(const FParams &p, int u, float2 
   __syncthreads();
 
   auto one_pol = [&](const int pol) {
-    // An opaque copy of the thread index (and an opaque zero offset) per
-    // polarisation keeps the compiler from hoisting the per-thread LDS and
-    // global addresses, the rotator, twiddle and table reads and their
-    // products out of the loop, where they would hold ~100 registers over
-    // both FFTs.
+    // An opaque copy of the thread index (and an opaque zero offset) in
+    // each polarisation's copy of this code keeps the compiler from sharing
+    // the per-thread LDS and global addresses, the rotator, twiddle and
+    // table reads and their products between the two copies, where they
+    // would hold ~100 registers over the first FFT.
     int z = 0, j = threadIdx.x;
     if constexpr (BOTH) asm volatile("" : "+s"(z), "+v"(j));
     const float2 *sub = stab + z;
