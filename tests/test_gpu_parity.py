@@ -59,6 +59,20 @@ def run_oracle(w, data, poly, offs, autos=True, mode="f32", quirk=True):
                                               (8192, True), (8192, False), (16384, False),
                                               (16384, True), (32768, True)])
 def test_fstage_matches_oracle(gpu, nchan, is_complex):
+    check_fstage_vs_oracle(nchan, is_complex)
+
+
+@pytest.mark.parametrize("nchan,is_complex", [(1024, False), (2048, False), (4096, False),
+                                              (2048, True), (4096, True), (8192, True)])
+def test_register_fstage_matches_oracle_with_invalid_ffts(gpu, nchan, is_complex):
+    """The register kernel (fblock = L/32) at 2048..8192 points, where one
+    workgroup takes both polarisations of a station and FFT: spectra per
+    polarisation against the oracle, and invalid FFTs zero in both rows."""
+    L = nchan if is_complex else 2 * nchan
+    check_fstage_vs_oracle(nchan, is_complex, fblock=L // 32)
+
+
+def check_fstage_vs_oracle(nchan, is_complex, **kw):
     w, data, poly, offs = random_case(3, nchan, 40, seed=nchan + is_complex, is_complex=is_complex,
                                       bw=128e6 if is_complex else 64e6)
     # station 1: large net delay so FFT 0 is out of the buffer; station 2:
@@ -67,7 +81,7 @@ def test_fstage_matches_oracle(gpu, nchan, is_complex):
     offs = offs.copy()
     offs[1] = poly[1][2] - 5.3 * Ts
     offs[2] = poly[2][2] + 3.6 * Ts
-    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex)
+    k = gcorr_amd.FxKernel(w.nant, w.nchan, w.numffts, w.nbit, w.lo, w.bw, w.is_complex, **kw)
     k.setInputData(data)
     k.setDelays(poly, offs)
     spec, valid = k.fstage(0, w.numffts)
