@@ -1,6 +1,6 @@
 // Per-subband launch of the register/plane F-stage (fstage_big.hpp): one
-// workgroup of L/32 threads per (station, FFT) up to 8192 points, per
-// (station, FFT, pol) above.  The only F-stage for
+// workgroup of L/32 threads per (station, FFT) for real input up to 8192
+// points, per (station, FFT, pol) otherwise.  The only F-stage for
 // L = 16384, 32768; at L = 2048..8192 an alternative to fstage_item that the
 // autotuner weighs.
 #include "fstage_big.hpp"
@@ -10,13 +10,14 @@ namespace {
 
 using namespace fs;
 
-// Up to 8192 points a workgroup takes both polarisations of one (station,
-// FFT) in turn, so the delay split and the rotator tables are built once for
-// two FFTs; the two FFTs fit 128 registers (4 waves per SIMD).  At 16384 and
-// 32768 points (512 and 1024 threads) the second copy of the FFT spills, and
-// a workgroup takes one polarisation.
-template <int L>
-constexpr bool big_both() { return L <= 8192; }
+// For real input up to 8192 points a workgroup takes both polarisations of
+// one (station, FFT) in turn, so the delay split and the rotator tables are
+// built once for two FFTs; the two FFTs fit 128 registers (4 waves per
+// SIMD).  At 16384 and 32768 points (512 and 1024 threads), and for complex
+// input, whose two copies hold full 32-bit sample words, the second copy
+// spills, and a workgroup takes one polarisation.
+template <int L, bool CPLX>
+constexpr bool big_both() { return !CPLX && L <= 8192; }
 
 template <int L, bool CPLX, int OUT>
 __global__ void __launch_bounds__((BG<L, CPLX>::T)) __attribute__((amdgpu_waves_per_eu(4)))
@@ -26,13 +27,13 @@ fstage_big_kernel(FParams p) {
   // exactly; in the second form the two polarisations of one (station,
   // FFT) are consecutive units on one XCD, so the second finds the input
   // words in L2
-  fstage_big_item<L, CPLX, OUT, big_both<L>()>(p, (int)xcd_order(blockIdx.x, gridDim.x), smem);
+  fstage_big_item<L, CPLX, OUT, big_both<L, CPLX>()>(p, (int)xcd_order(blockIdx.x, gridDim.x), smem);
 }
 
 template <int L, bool CPLX>
 hipError_t launch_big(const FParams &p, hipStream_t s) {
   using G = BG<L, CPLX>;
-  const long long items = (big_both<L>() ? 1LL : 2LL) * p.nant * p.nt;
+  const long long items = (big_both<L, CPLX>() ? 1LL : 2LL) * p.nant * p.nt;
   if (items <= 0) return hipSuccess;
   if (items > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 grid((unsigned)items), block(G::T);
