@@ -65,9 +65,10 @@ def test_fstage_matches_oracle(gpu, nchan, is_complex):
 @pytest.mark.parametrize("nchan,is_complex", [(1024, False), (2048, False), (4096, False),
                                               (2048, True), (4096, True), (8192, True)])
 def test_register_fstage_matches_oracle_with_invalid_ffts(gpu, nchan, is_complex):
-    """The register kernel (fblock = L/32) at 2048..8192 points, where one
-    workgroup takes both polarisations of a station and FFT: spectra per
-    polarisation against the oracle, and invalid FFTs zero in both rows."""
+    """The register kernel (fblock = L/32) at 2048..8192 points, where for
+    real input one workgroup takes both polarisations of a station and FFT
+    (complex input: one each): spectra per polarisation against the oracle,
+    and invalid FFTs zero in both rows."""
     L = nchan if is_complex else 2 * nchan
     check_fstage_vs_oracle(nchan, is_complex, fblock=L // 32)
 
@@ -473,8 +474,8 @@ def test_launch_shape_variants_match_oracle(gpu, fblock_mult, xtile_mult):
                                               (2048, True), (4096, True), (8192, True)])
 def test_long_fft_kernel_at_short_lengths(gpu, nchan, is_complex):
     """At FFT lengths 2048..8192 the register kernel (fblock = L/32; both
-    polarisations of a station and FFT per workgroup) is an alternative the
-    autotuner can choose."""
+    polarisations of a station and FFT per workgroup for real input) is an
+    alternative the autotuner can choose."""
     w, data, poly, offs = random_case(3, nchan, 40, seed=15 + nchan, is_complex=is_complex,
                                       bw=128e6 if is_complex else 64e6)
     L = w.fftchannels
